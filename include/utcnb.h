@@ -1,0 +1,226 @@
+/*
+ * utcnb.h -- C ABI of the MI355X (gfx950) nbody library.
+ *
+ * This is the drop-in boundary for the UTC nbody benchapp's hot path: the
+ * all-pairs softened-gravity acceleration and the damped kick-drift update.
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (0 = success, negative = error; see utcnb_last_error()), and never lets a
+ * C++ exception cross the boundary.  Citations are relative to the reference's
+ * utc-0.1/ tree.
+ *
+ * Memory: the caller owns every host buffer and every pointer it passes in;
+ * a system handle owns the device buffers it allocates.  A handle is bound to
+ * the HIP stream given at creation; calls on one handle must not overlap.
+ *
+ * Layouts are the reference's: positions T[4N] = (x,y,z,m) per body,
+ * velocities T[4N] = (vx,vy,vz,w) per body (benchapps/nbody/seq/nbody.h:125-134).
+ */
+#ifndef UTCNB_H_
+#define UTCNB_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define UTCNB_OK 0
+#define UTCNB_EINVAL -1   /* bad argument (size, pointer, enum, value) */
+#define UTCNB_EHIP -2     /* a HIP runtime call failed */
+#define UTCNB_ENOMEM -3   /* device or host allocation failed */
+#define UTCNB_ENODEV -4   /* no usable gfx950 device */
+#define UTCNB_ESTATE -5   /* call out of order (e.g. step before set_state) */
+
+/* ---- enums -------------------------------------------------------------- */
+/* Arithmetic of the pair interaction and of the integrator state. */
+#define UTCNB_PREC_REFERENCE 0 /* the reference's own arithmetic in the state
+                                  dtype: sqrt then divide, no fused multiply-add,
+                                  j summed in order 0..N-1; bit-identical to
+                                  benchapps/nbody/seq/bodysystem.{h:70-96,cc:103-181} */
+#define UTCNB_PREC_FP32 1      /* plain FP32: v_rsq + Newton step, FMA, FP32 sums
+                                  and FP32 state */
+#define UTCNB_PREC_MIXED 2     /* FP32 pairs with compensated per-body sums;
+                                  FP64 state and integrator */
+#define UTCNB_PREC_FP64 3      /* FP64 pairs (correctly rounded sqrt/divide),
+                                  compensated FP64 sums, FP64 state */
+
+/* Kick factor applied to the summed field (benchapps/nbody/seq/bodysystem.cc:149,161-163). */
+#define UTCNB_KICK_REFERENCE 0 /* v += (a * pos.w) * dt   (the reference: pos.w = m_i) */
+#define UTCNB_KICK_PHYSICAL 1  /* v += a * dt */
+
+#define UTCNB_F32 0
+#define UTCNB_F64 1
+
+#define UTCNB_HOST 0
+#define UTCNB_DEVICE 1
+
+/* config flags */
+#define UTCNB_FLAG_NO_NEWTON 1 /* FP32/MIXED: use v_rsq_f32 without the Newton step */
+
+/* ---- kernel-level entry points (device pointers, reference layout) ------ */
+/*
+ * One fused step in the reference's arithmetic: forces from pos_old for every
+ * body, then v = (v + (a*pos.w)*dt)*damping, x = x + v*dt into pos_new (pos.w
+ * copied) and vel (in place).
+ * Replaces the launch of _integrateNBodySystem_kernel<T> /
+ * _integrateNBodySystemSmall_kernel<T> at
+ * benchapps/nbody/utc/sgpu/body_task_sgpu.cu:117-154 (kernels:
+ * benchapps/nbody/cuda/bodysystem_kernel.cu:87-138,186-243), with the same
+ * argument meaning.  `blocksize` is the workgroup size (64..1024, multiple of
+ * 64; 0 = 256).  Unlike the reference, numBodies need not be a multiple of
+ * blocksize.  Results are bit-identical to the reference's CPU path; the
+ * reference's split-j "Small" variant changes the summation order and is not
+ * reproduced.  pos_old and pos_new must not alias.
+ */
+int utcnb_integrate_f32(const float* pos_old, float* pos_new, float* vel, int numBodies,
+                        float deltaTime, float softeningSquared, float damping, int blocksize,
+                        void* stream);
+int utcnb_integrate_f64(const double* pos_old, double* pos_new, double* vel, int numBodies,
+                        double deltaTime, double softeningSquared, double damping,
+                        int blocksize, void* stream);
+
+/* ---- system handle ------------------------------------------------------ */
+typedef struct utcnb_system utcnb_system;
+
+typedef struct {
+    int32_t n;           /* bodies, >= 1 */
+    int32_t precision;   /* UTCNB_PREC_* */
+    int32_t kick;        /* UTCNB_KICK_* */
+    int32_t state_dtype; /* UTCNB_PREC_REFERENCE only: UTCNB_F32 or UTCNB_F64 */
+    int32_t flags;       /* UTCNB_FLAG_* */
+    int32_t tile;        /* force-kernel tile id, 0 = tuned default (utcnb_autotune) */
+    double eps2;         /* softening squared, > 0 unless precision is REFERENCE */
+    double damping;      /* velocity damping per step (reference default 1.0) */
+    double dt;           /* time step */
+} utcnb_config;
+
+/*
+ * Create a system: allocates the double-buffered device state.  Replaces the
+ * GpuData allocations and kernel-shape selection in
+ * BodySystemSGPU<T>::runImpl, benchapps/nbody/utc/sgpu/body_task_sgpu.cu:100-127
+ * (GpuData: include/gpu/GpuData.inc:69-102).  `stream` is a hipStream_t (NULL =
+ * the null stream).
+ */
+int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out);
+/* Free device buffers; replaces the GpuData destructors (include/gpu/GpuData.inc:130-157). */
+int utcnb_destroy(utcnb_system* sys);
+
+/*
+ * Upload state (pos4 and vel4, n*4 values of `dtype`, converted to the
+ * system's state type).  where = UTCNB_HOST for host pointers, UTCNB_DEVICE for
+ * device pointers.  Replaces GpuData::initH + sync (H2D) at
+ * benchapps/nbody/utc/sgpu/body_task_sgpu.cu:102-111.  vel4 may be NULL
+ * (zero velocities, w = 1).
+ */
+int utcnb_set_state(utcnb_system* sys, const void* pos4, const void* vel4, int dtype,
+                    int where);
+/* Download state; replaces GpuData::sync (D2H) + fetch at
+ * benchapps/nbody/utc/sgpu/body_task_sgpu.cu:161-170.  Either pointer may be NULL.
+ * Synchronizes the system's stream when where = UTCNB_HOST. */
+int utcnb_get_state(utcnb_system* sys, void* pos4, void* vel4, int dtype, int where);
+
+/* Advance nsteps steps (asynchronous on the system's stream); replaces the
+ * launch loop of benchapps/nbody/utc/sgpu/body_task_sgpu.cu:130-158. */
+int utcnb_step(utcnb_system* sys, int nsteps);
+/* Wait for the system's stream. */
+int utcnb_sync(utcnb_system* sys);
+
+/* Acceleration field of the current state, as the system's force kernel
+ * computes it (FP32 hi+lo for MIXED), written as FP64 acc3[3n] (host). */
+int utcnb_accel(utcnb_system* sys, double* acc3);
+/* FP64 diagnostics of the current state: kinetic 1/2 sum m|v|^2, potential
+ * -sum_{i<j} m_i m_j / sqrt(r^2+eps2) and momentum sum m v (host outputs). */
+int utcnb_diagnostics(utcnb_system* sys, double* kinetic, double* potential,
+                      double* momentum3);
+
+/*
+ * Forward `nsteps` steps from the current state, then the adjoint for
+ * L = (1/n) sum_i |x_i(T) - t_i|^2 back to that state.  FP64 checkpoints of
+ * (x, v) are kept every `ckpt_every` steps and the steps in between are
+ * recomputed during the reverse sweep.  Precision MIXED (FP32 pair VJP) or
+ * FP64.  target3: host FP64 [3n].  Outputs (host, FP64): *loss, dx0[3n],
+ * dv0[3n], dm[n].  The system ends in the final (step nsteps) state.
+ */
+int utcnb_adjoint(utcnb_system* sys, int nsteps, int ckpt_every, const double* target3,
+                  double* loss, double* dx0, double* dv0, double* dm);
+
+/* Total time (ms) of the force kernels launched by the last utcnb_step /
+ * utcnb_adjoint call on this handle, measured with HIP events on its stream;
+ * *launches receives their count.  Synchronizes the stream. */
+int utcnb_last_kernel_ms(utcnb_system* sys, double* ms, int* launches);
+
+/* ---- batched systems ---------------------------------------------------- */
+typedef struct utcnb_batch utcnb_batch;
+
+/*
+ * Many independent systems of uneven size in one device allocation.
+ * sizes[nsys]; every system shares cfg's precision (MIXED or FP64), kick,
+ * eps2, damping and dt (cfg->n is ignored).  The reference's analog is
+ * benchapps/nbody/utc/nbody_mtask.cc:185-231 (one task and stream per system).
+ */
+int utcnb_batch_create(int nsys, const int32_t* sizes, const utcnb_config* cfg, void* stream,
+                       utcnb_batch** out);
+int utcnb_batch_destroy(utcnb_batch* b);
+/* Concatenated FP64 host state, systems in order: pos4/vel4 [4 * sum(sizes)]. */
+int utcnb_batch_set_state(utcnb_batch* b, const double* pos4, const double* vel4);
+int utcnb_batch_get_state(utcnb_batch* b, double* pos4, double* vel4);
+/*
+ * mode 0: one captured HIP graph of persistent step kernels over all systems
+ *         (replayed per call; captured on first use);
+ * mode 1: per-system launches (the baseline).
+ * When target3 is non-NULL the adjoint of the same loss as utcnb_adjoint runs
+ * after the forward steps (checkpoint every ckpt_every) and losses[nsys],
+ * dx0/dv0 [3 * sum(sizes)], dm [sum(sizes)] receive per-system results.
+ * *ms receives the wall time of the call measured with HIP events.
+ */
+int utcnb_batch_run(utcnb_batch* b, int nsteps, int mode, int ckpt_every,
+                    const double* target3, double* losses, double* dx0, double* dv0,
+                    double* dm, double* ms);
+
+/* ---- tile autotuner ----------------------------------------------------- */
+/*
+ * Time the force-kernel tiles for an n-body system in `precision` on the
+ * current device and return the fastest tile id (usable as utcnb_config.tile).
+ * Results are cached in memory and, when cache_path is non-NULL, in a text file
+ * keyed by device and problem class.
+ */
+int utcnb_autotune(int n, int precision, const char* cache_path, int* best_tile,
+                   double* best_ms);
+/* Number of tiles and a description of tile id t (static string). */
+int utcnb_tile_count(void);
+const char* utcnb_tile_name(int tile);
+
+/* ---- input generators (host) -------------------------------------------- */
+/* xoshiro256** seeded by splitmix64 (published algorithms of Blackman & Vigna). */
+void utcnb_rng_seed(uint64_t seed, uint64_t state[4]);
+uint64_t utcnb_rng_next(uint64_t state[4]);
+uint64_t utcnb_splitmix64(uint64_t* x);
+/*
+ * Fill pos4/vel4 (FP64 [4n]) and, for TARGETS, out3 [3n]:
+ *   UTCNB_GEN_CUBE      x ~ U[-1,1]^3, v ~ N(0, p0^2), m = 1/n, w = 1
+ *   UTCNB_GEN_CUBE_LOGM as CUBE with m ~ exp(U[ln p1, ln p2]) / n
+ *   UTCNB_GEN_PLUMMER   Plummer sphere, a = 1, total mass 1, equal masses,
+ *                       Aarseth-Henon-Wielen (1974) sampler, mass cut 0.999,
+ *                       centre of mass and mean velocity removed
+ *   UTCNB_GEN_TARGETS   out3 ~ N(0, p0^2)
+ */
+#define UTCNB_GEN_CUBE 0
+#define UTCNB_GEN_CUBE_LOGM 1
+#define UTCNB_GEN_PLUMMER 2
+#define UTCNB_GEN_TARGETS 3
+int utcnb_generate(int kind, int n, uint64_t seed, double p0, double p1, double p2,
+                   double* pos4, double* vel4, double* out3);
+
+/* ---- misc --------------------------------------------------------------- */
+/* Last error message of the calling thread ("" if none). */
+const char* utcnb_last_error(void);
+/* Library version string. */
+const char* utcnb_version(void);
+/* Device name and compute-unit count of the current device (0 on success). */
+int utcnb_device_info(char* name, int name_len, int* cu_count, int* clock_khz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UTCNB_H_ */

@@ -1,0 +1,203 @@
+"""GPU parity of the forward path (force + damped kick-drift) against the oracle.
+
+Reference arithmetic (UTCNB_PREC_REFERENCE) must be bit-identical to the
+reference's CPU path; the fast FP32 / mixed / FP64 paths must meet the stated
+tolerances against the FP64 oracle.
+"""
+import hashlib
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import utc_amd as U
+from nbcases import config1, plummer, relnorm
+from oracle import oracle as O
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+SEQ_SHA = "1484534f18e875397249e74b1a319161a7b0d70307ee4dffc1ea1683082ce1d3"
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def fixture_run_frames(dtype):
+    """benchapps/nbody/seq/nbody_main.cc as run for the golden file:
+    -n 1024 -l 100, preset 0 (dt 0.016, softening 0.1, damping 1), SHELL."""
+    n = 1024
+    prm = O.demo_params(0, n)
+    pos, vel = O.randomize(1, n, prm["clusterScale"], prm["velocityScale"], dtype=dtype)
+    T = np.float32 if dtype == np.float32 else np.float64
+    eps2 = T(prm["softening"]) * T(prm["softening"])
+    return n, prm, pos, vel, eps2
+
+
+def test_reference_f32_reproduces_golden_file(tmp_path):
+    n, prm, pos, vel, eps2 = fixture_run_frames(np.float32)
+    say(f"case: REFERENCE f32 golden run n={n} steps=100")
+    with U.System(n, U.PREC_REFERENCE, float(eps2), float(prm["timestep"]), float(prm["damping"]),
+                  kick=U.KICK_REFERENCE, state_dtype=U.F32) as s:
+        s.set_state(pos, vel)
+        s.step(100)
+        p1, _ = s.get_state(np.float32)
+    frames = np.stack([pos, p1])
+    out = tmp_path / "out.txt"
+    O.write_frames(out, frames, prm["timestep"])
+    data = out.read_bytes()
+    assert data == (GOLDEN / "seq_nbody_outpos.txt").read_bytes()
+    assert hashlib.sha256(data).hexdigest() == SEQ_SHA
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_reference_config1_bit_exact(dtype):
+    c = config1()
+    T = dtype
+    pos = c["pos"].astype(T)
+    vel = c["vel"].astype(T)
+    eps2 = T(c["eps"]) * T(c["eps"])
+    say(f"case: REFERENCE {np.dtype(T).name} config1 n={len(pos)}")
+    with U.System(len(pos), U.PREC_REFERENCE, float(eps2), float(T(c["dt"])), 1.0,
+                  kick=U.KICK_REFERENCE, state_dtype=U.F32 if T == np.float32 else U.F64) as s:
+        s.set_state(pos, vel)
+        s.step(c["steps"])
+        gp, gv = s.get_state(T)
+    if T == np.float32:
+        op, ov = O.run_f32(pos, vel, c["steps"], eps2, 1.0, T(c["dt"]), nthreads=8)
+    else:
+        op, ov = O.run_f64(pos, vel, c["steps"], eps2, 1.0, c["dt"])
+    assert np.array_equal(gp, op), f"max diff {np.abs(gp - op).max()}"
+    assert np.array_equal(gv, ov)
+
+
+def test_reference_f32_field_matches_oracle_gravitation():
+    n, prm, pos, vel, eps2 = fixture_run_frames(np.float32)
+    say("case: REFERENCE f32 field n=1024")
+    with U.System(n, U.PREC_REFERENCE, float(eps2), 0.016, 1.0, kick=U.KICK_REFERENCE,
+                  state_dtype=U.F32) as s:
+        s.set_state(pos, vel)
+        a = s.accel()
+    f = O.gravitation_f32(pos, eps2)
+    assert np.array_equal(a.astype(np.float32), f)
+
+
+def test_kernel_level_entry_point_matches_oracle():
+    """utcnb_integrate_f32 on device pointers, like the reference's kernel launch."""
+    import torch
+    c = config1(n=2048)
+    pos = c["pos"].astype(np.float32)
+    vel = c["vel"].astype(np.float32)
+    eps2 = np.float32(c["eps"]) * np.float32(c["eps"])
+    dt = np.float32(c["dt"])
+    say("case: utcnb_integrate_f32 n=2048")
+    dev = torch.device("cuda:0")
+    p_old = torch.from_numpy(pos).to(dev)
+    p_new = torch.empty_like(p_old)
+    v = torch.from_numpy(vel).to(dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    for blocksize in (0, 64, 256, 1024):
+        p_new.zero_()
+        v.copy_(torch.from_numpy(vel))
+        rc = U.lib().utcnb_integrate_f32(p_old.data_ptr(), p_new.data_ptr(), v.data_ptr(), len(pos),
+                                         float(dt), float(eps2), 1.0, blocksize, stream)
+        U.check(rc, "utcnb_integrate_f32")
+        torch.cuda.synchronize()
+        op, ov = O.run_f32(pos, vel, 1, eps2, 1.0, dt, nthreads=8)
+        assert np.array_equal(p_new.cpu().numpy(), op)
+        assert np.array_equal(v.cpu().numpy(), ov)
+
+
+def test_fp32_config1_within_tolerance_of_reference():
+    """Plain fast FP32 (rsq, FMA) vs the reference CPU path on config 1:
+    max |dx| <= 1e-6 and relative norm <= 1e-6."""
+    c = config1()
+    pos = c["pos"].astype(np.float32)
+    vel = c["vel"].astype(np.float32)
+    eps2 = np.float32(c["eps"]) * np.float32(c["eps"])
+    dt = np.float32(c["dt"])
+    op, _ = O.run_f32(pos, vel, c["steps"], eps2, 1.0, dt, nthreads=8)
+    for flags in (0, U.FLAG_NO_NEWTON):
+        say(f"case: FP32 config1 flags={flags}")
+        with U.System(len(pos), U.PREC_FP32, float(eps2), float(dt), 1.0,
+                      kick=U.KICK_REFERENCE, flags=flags) as s:
+            s.set_state(pos, vel)
+            s.step(c["steps"])
+            gp, _ = s.get_state(np.float32)
+        d = np.abs(gp[:, :3].astype(np.float64) - op[:, :3])
+        assert d.max() <= 1e-6, d.max()
+        assert relnorm(gp[:, :3], op[:, :3]) <= 1e-6
+        assert np.array_equal(gp[:, 3], op[:, 3])
+
+
+def test_fp64_field_matches_cpu_fp64():
+    c = plummer(2048, eps=0.005)
+    say("case: FP64 field n=2048")
+    a64 = O.accel_f64(c["pos"], c["eps"] ** 2)
+    with U.System(2048, U.PREC_FP64, c["eps"] ** 2, c["dt"]) as s:
+        s.set_state(c["pos"], c["vel"])
+        a = s.accel()
+    assert relnorm(a, a64) <= 1e-14
+    assert np.abs(a - a64).max() <= 1e-12 * np.abs(a64).max()
+
+
+@pytest.mark.parametrize("prec", [U.PREC_FP32, U.PREC_MIXED])
+@pytest.mark.parametrize("tile", list(range(1, 11)))
+def test_every_tile_field_accuracy(prec, tile):
+    n = 3000  # not a multiple of any tile: exercises clamped targets and padding
+    c = plummer(n, eps=0.005)
+    a64 = O.accel_f64(c["pos"], c["eps"] ** 2)
+    say(f"case: field prec={prec} tile={tile} n={n}")
+    with U.System(n, prec, c["eps"] ** 2, c["dt"], tile=tile) as s:
+        s.set_state(c["pos"], c["vel"])
+        a = s.accel()
+    # the FP32 positions alone perturb close pairs by ~u|x|/r
+    assert relnorm(a, a64) <= 2e-6
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 1000, 4097])
+def test_odd_sizes(n):
+    c = plummer(n, eps=0.01, dt=1e-3, steps=3)
+    say(f"case: odd size n={n}")
+    p64, v64 = O.steps_f64(c["pos"], c["vel"], 3, c["eps"] ** 2, c["dt"])
+    for prec in (U.PREC_FP32, U.PREC_MIXED, U.PREC_FP64):
+        with U.System(n, prec, c["eps"] ** 2, c["dt"]) as s:
+            s.set_state(c["pos"], c["vel"])
+            s.step(3)
+            gp, gv = s.get_state()
+        tol = 1e-12 if prec == U.PREC_FP64 else 1e-6
+        assert np.abs(gp - p64).max() <= tol * max(1.0, np.abs(p64).max()), (prec, n)
+
+
+def test_mixed_and_fp64_trajectory_config2_small():
+    """Config-2 physics at N=4096: mixed path final positions vs the FP64 oracle."""
+    c = plummer(4096)
+    say("case: MIXED/FP64 trajectory n=4096 steps=20")
+    p64, v64 = O.steps_f64(c["pos"], c["vel"], c["steps"], c["eps"] ** 2, c["dt"])
+    res = {}
+    for prec in (U.PREC_FP32, U.PREC_MIXED, U.PREC_FP64):
+        with U.System(4096, prec, c["eps"] ** 2, c["dt"]) as s:
+            s.set_state(c["pos"], c["vel"])
+            s.step(c["steps"])
+            gp, _ = s.get_state()
+        res[prec] = relnorm(gp[:, :3], p64[:, :3])
+    say(f"final-position rel err: {res}")
+    assert res[U.PREC_FP64] <= 1e-13
+    assert res[U.PREC_MIXED] <= 1e-8
+    assert res[U.PREC_FP32] <= 1e-6
+
+
+def test_diagnostics_match_oracle():
+    c = plummer(2048, eps=0.01)
+    say("case: diagnostics n=2048")
+    K, W, P = O.energy_f64(c["pos"], c["vel"], c["eps"] ** 2)
+    for prec in (U.PREC_MIXED, U.PREC_FP64):
+        with U.System(2048, prec, c["eps"] ** 2, c["dt"]) as s:
+            s.set_state(c["pos"], c["vel"])
+            k, w, p = s.diagnostics()
+        assert abs(k - K) <= 1e-13 * abs(K)
+        assert abs(w - W) <= 1e-12 * abs(W)
+        assert np.abs(p - P).max() <= 1e-15

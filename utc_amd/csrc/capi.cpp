@@ -1,0 +1,578 @@
+// capi.cpp -- the C ABI (include/utcnb.h): system handles over the gfx950 kernels.
+//
+// A handle owns the device state of one system in the layout its precision
+// needs (citations relative to the reference's utc-0.1/ tree):
+//   REFERENCE f32/f64 : pos T4[2] (double buffered) + vel T4, like the
+//                       GpuData pair of benchapps/nbody/utc/sgpu/body_task_sgpu.cu:100-101
+//   FP32              : pos float4[2] padded to kPadJ + vel float4
+//   MIXED             : FP64 pos/vel double4 (updated in place) + FP32
+//                       snapshot float4[2] padded, read by the force kernel
+//   FP64              : pos double4[2] padded + vel double4
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace utcnb {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+}  // namespace utcnb
+
+using namespace utcnb;
+
+struct utcnb_system {
+    utcnb_config cfg{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int n = 0, n_pad = 0;
+    int cur = 0;
+    bool has_state = false;
+    StepArgs args{};
+    float4* p32[2] = {nullptr, nullptr};
+    float4* v32 = nullptr;
+    double4* p64[2] = {nullptr, nullptr};
+    double4* v64 = nullptr;
+    float4* snap[2] = {nullptr, nullptr};
+    double* scratch = nullptr;   // 3n doubles (field) / energy partials
+    size_t scratch_bytes = 0;
+    void* stage = nullptr;       // conversion staging, 4n * 8 bytes
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int last_launches = 0;
+    bool timed = false;
+    int cu_count = 256;
+};
+
+namespace {
+
+int check_device(int* dev, int* cus)
+{
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess)
+        return fail(UTCNB_ENODEV, "no HIP device available");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) != hipSuccess)
+        return fail(UTCNB_ENODEV, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(UTCNB_ENODEV, std::string("device is ") + prop.gcnArchName +
+                                      ", this library is built for gfx950 only");
+    *dev = d;
+    if (cus)
+        *cus = prop.multiProcessorCount;
+    return UTCNB_OK;
+}
+
+template <typename T>
+int dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0)
+        return UTCNB_OK;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(UTCNB_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return UTCNB_OK;
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p)
+        (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void release(utcnb_system* s)
+{
+    dfree(s->p32[0]);
+    dfree(s->p32[1]);
+    dfree(s->v32);
+    dfree(s->p64[0]);
+    dfree(s->p64[1]);
+    dfree(s->v64);
+    dfree(s->snap[0]);
+    dfree(s->snap[1]);
+    dfree(s->scratch);
+    if (s->stage)
+        (void)hipFree(s->stage);
+    s->stage = nullptr;
+    if (s->ev0)
+        (void)hipEventDestroy(s->ev0);
+    if (s->ev1)
+        (void)hipEventDestroy(s->ev1);
+}
+
+bool is_f32_state(const utcnb_system* s)
+{
+    return s->cfg.precision == UTCNB_PREC_FP32 ||
+           (s->cfg.precision == UTCNB_PREC_REFERENCE && s->cfg.state_dtype == UTCNB_F32);
+}
+
+// Current FP64 position / velocity arrays (null when the state is FP32).
+double4* cur_p64(utcnb_system* s)
+{
+    return s->cfg.precision == UTCNB_PREC_MIXED ? s->p64[0] : s->p64[s->cur];
+}
+
+hipMemcpyKind kind_of(int where, bool to_device)
+{
+    if (where == UTCNB_DEVICE)
+        return hipMemcpyDeviceToDevice;
+    return to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+}
+
+// Bring an input array (n*4 of dtype, host or device) to the device as `T4`.
+template <typename T4>
+int upload4(utcnb_system* s, const void* src, int dtype, int where, T4* dst)
+{
+    const size_t n4 = 4 * (size_t)s->n;
+    constexpr bool dst_f32 = sizeof(T4) == sizeof(float4);
+    const bool same = (dtype == UTCNB_F32) == dst_f32;
+    if (same) {
+        UTCNB_HIP(hipMemcpyAsync(dst, src, n4 * (dst_f32 ? 4 : 8), kind_of(where, true),
+                                 s->stream));
+        return UTCNB_OK;
+    }
+    const size_t sb = n4 * (dtype == UTCNB_F32 ? 4 : 8);
+    UTCNB_HIP(hipMemcpyAsync(s->stage, src, sb, kind_of(where, true), s->stream));
+    hipError_t e;
+    if constexpr (dst_f32)
+        e = launch_f64_to_f32((const double4*)s->stage, (float4*)dst, s->n, s->stream);
+    else
+        e = launch_f32_to_f64((const float4*)s->stage, (double4*)dst, s->n, s->stream);
+    UTCNB_HIP(e);
+    return UTCNB_OK;
+}
+
+template <typename T4>
+int download4(utcnb_system* s, const T4* src, void* dst, int dtype, int where)
+{
+    const size_t n4 = 4 * (size_t)s->n;
+    constexpr bool src_f32 = sizeof(T4) == sizeof(float4);
+    const bool same = (dtype == UTCNB_F32) == src_f32;
+    if (same) {
+        UTCNB_HIP(hipMemcpyAsync(dst, src, n4 * (src_f32 ? 4 : 8), kind_of(where, false),
+                                 s->stream));
+        return UTCNB_OK;
+    }
+    hipError_t e;
+    if constexpr (src_f32)
+        e = launch_f32_to_f64((const float4*)src, (double4*)s->stage, s->n, s->stream);
+    else
+        e = launch_f64_to_f32((const double4*)src, (float4*)s->stage, s->n, s->stream);
+    UTCNB_HIP(e);
+    const size_t db = n4 * (dtype == UTCNB_F32 ? 4 : 8);
+    UTCNB_HIP(hipMemcpyAsync(dst, s->stage, db, kind_of(where, false), s->stream));
+    return UTCNB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* utcnb_last_error(void) { return g_last_error.c_str(); }
+
+const char* utcnb_version(void) { return "utcnb 0.1 gfx950"; }
+
+int utcnb_device_info(char* name, int name_len, int* cu_count, int* clock_khz)
+{
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess)
+        return fail(UTCNB_ENODEV, "no HIP device available");
+    hipDeviceProp_t prop;
+    UTCNB_HIP(hipGetDeviceProperties(&prop, d));
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s", prop.gcnArchName);
+    }
+    if (cu_count)
+        *cu_count = prop.multiProcessorCount;
+    if (clock_khz)
+        *clock_khz = prop.clockRate;
+    return UTCNB_OK;
+}
+
+int utcnb_integrate_f32(const float* pos_old, float* pos_new, float* vel, int numBodies,
+                        float deltaTime, float softeningSquared, float damping, int blocksize,
+                        void* stream)
+{
+    if (!pos_old || !pos_new || !vel || numBodies <= 0)
+        return fail(UTCNB_EINVAL, "utcnb_integrate_f32: null pointer or numBodies <= 0");
+    if (pos_old == pos_new)
+        return fail(UTCNB_EINVAL, "utcnb_integrate_f32: pos_old and pos_new alias");
+    if (blocksize != 0 && (blocksize < 64 || blocksize > 1024 || blocksize % 64 != 0))
+        return fail(UTCNB_EINVAL, "utcnb_integrate_f32: blocksize must be 64..1024, multiple of 64");
+    UTCNB_HIP(launch_ref_step_f32((const float4*)pos_old, (float4*)pos_new, (float4*)vel,
+                                  numBodies, deltaTime, softeningSquared, damping, blocksize,
+                                  (hipStream_t)stream));
+    return UTCNB_OK;
+}
+
+int utcnb_integrate_f64(const double* pos_old, double* pos_new, double* vel, int numBodies,
+                        double deltaTime, double softeningSquared, double damping,
+                        int blocksize, void* stream)
+{
+    if (!pos_old || !pos_new || !vel || numBodies <= 0)
+        return fail(UTCNB_EINVAL, "utcnb_integrate_f64: null pointer or numBodies <= 0");
+    if (pos_old == pos_new)
+        return fail(UTCNB_EINVAL, "utcnb_integrate_f64: pos_old and pos_new alias");
+    if (blocksize != 0 && (blocksize < 64 || blocksize > 1024 || blocksize % 64 != 0))
+        return fail(UTCNB_EINVAL, "utcnb_integrate_f64: blocksize must be 64..1024, multiple of 64");
+    UTCNB_HIP(launch_ref_step_f64((const double4*)pos_old, (double4*)pos_new, (double4*)vel,
+                                  numBodies, deltaTime, softeningSquared, damping, blocksize,
+                                  (hipStream_t)stream));
+    return UTCNB_OK;
+}
+
+int utcnb_tile_count(void) { return tile_count(); }
+
+const char* utcnb_tile_name(int tile)
+{
+    if (tile < 1 || tile > tile_count())
+        return "";
+    return tile_get(tile - 1).name;
+}
+
+int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
+{
+    if (!cfg || !out)
+        return fail(UTCNB_EINVAL, "utcnb_create: null argument");
+    *out = nullptr;
+    const utcnb_config c = *cfg;
+    if (c.n < 1 || (int64_t)c.n > (int64_t)1 << 30)
+        return fail(UTCNB_EINVAL, "utcnb_create: n must be in [1, 2^30]");
+    if (c.precision < UTCNB_PREC_REFERENCE || c.precision > UTCNB_PREC_FP64)
+        return fail(UTCNB_EINVAL, "utcnb_create: unknown precision");
+    if (c.kick != UTCNB_KICK_REFERENCE && c.kick != UTCNB_KICK_PHYSICAL)
+        return fail(UTCNB_EINVAL, "utcnb_create: unknown kick mode");
+    if (c.precision == UTCNB_PREC_REFERENCE) {
+        if (c.state_dtype != UTCNB_F32 && c.state_dtype != UTCNB_F64)
+            return fail(UTCNB_EINVAL, "utcnb_create: state_dtype must be F32 or F64");
+        if (c.kick != UTCNB_KICK_REFERENCE)
+            return fail(UTCNB_EINVAL, "utcnb_create: REFERENCE precision uses the reference kick");
+        if (!(c.eps2 >= 0.0))
+            return fail(UTCNB_EINVAL, "utcnb_create: eps2 must be >= 0");
+    } else if (!(c.eps2 > 0.0) || !std::isfinite(c.eps2)) {
+        return fail(UTCNB_EINVAL, "utcnb_create: eps2 must be finite and > 0");
+    }
+    if (!std::isfinite(c.dt) || !std::isfinite(c.damping))
+        return fail(UTCNB_EINVAL, "utcnb_create: dt and damping must be finite");
+    if (c.tile < 0 || c.tile > tile_count())
+        return fail(UTCNB_EINVAL, "utcnb_create: tile out of range");
+
+    int dev = 0, cus = 256;
+    int rc = check_device(&dev, &cus);
+    if (rc)
+        return rc;
+
+    utcnb_system* s = new (std::nothrow) utcnb_system();
+    if (!s)
+        return fail(UTCNB_ENOMEM, "utcnb_create: out of host memory");
+    s->cfg = c;
+    s->stream = (hipStream_t)stream;
+    s->device = dev;
+    s->cu_count = cus;
+    s->n = c.n;
+    const bool fast = c.precision != UTCNB_PREC_REFERENCE;
+    s->n_pad = fast ? (int)pad_j(c.n) : c.n;
+    const size_t np = (size_t)s->n_pad, n = (size_t)c.n;
+
+    s->args.n = c.n;
+    s->args.n_pad = s->n_pad;
+    s->args.eps2 = c.eps2;
+    s->args.eps2f = (float)c.eps2;
+    s->args.dt = c.dt;
+    s->args.damping = c.damping;
+    s->args.kick = c.kick;
+    s->args.newton = (c.flags & UTCNB_FLAG_NO_NEWTON) == 0;
+    s->args.tile = c.tile > 0 ? c.tile - 1 : tile_default(c.n, cus);
+
+    switch (c.precision) {
+    case UTCNB_PREC_REFERENCE:
+        if (c.state_dtype == UTCNB_F32)
+            rc = dalloc(&s->p32[0], np) || dalloc(&s->p32[1], np) || dalloc(&s->v32, n);
+        else
+            rc = dalloc(&s->p64[0], np) || dalloc(&s->p64[1], np) || dalloc(&s->v64, n);
+        break;
+    case UTCNB_PREC_FP32:
+        rc = dalloc(&s->p32[0], np) || dalloc(&s->p32[1], np) || dalloc(&s->v32, n);
+        break;
+    case UTCNB_PREC_MIXED:
+        rc = dalloc(&s->p64[0], n) || dalloc(&s->v64, n) || dalloc(&s->snap[0], np) ||
+             dalloc(&s->snap[1], np);
+        break;
+    case UTCNB_PREC_FP64:
+        rc = dalloc(&s->p64[0], np) || dalloc(&s->p64[1], np) || dalloc(&s->v64, n);
+        break;
+    }
+    if (!rc) {
+        s->scratch_bytes = sizeof(double) * (3 * n + 5 + 5 * ((n + 255) / 256));
+        rc = dalloc((char**)&s->scratch, s->scratch_bytes);
+    }
+    if (!rc) {
+        hipError_t e = hipMalloc(&s->stage, 4 * n * sizeof(double) + 4 * n * sizeof(double));
+        if (e != hipSuccess)
+            rc = fail(UTCNB_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    if (!rc && (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess))
+        rc = fail(UTCNB_EHIP, "hipEventCreate failed");
+    if (!rc && fast) {
+        hipError_t e = hipSuccess;
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+            if (s->p32[b])
+                e = launch_pad_f32(s->p32[b], c.n, s->n_pad, s->stream);
+            if (e == hipSuccess && s->snap[b])
+                e = launch_pad_f32(s->snap[b], c.n, s->n_pad, s->stream);
+            if (e == hipSuccess && s->p64[b] && c.precision == UTCNB_PREC_FP64)
+                e = launch_pad_f64(s->p64[b], c.n, s->n_pad, s->stream);
+        }
+        if (e != hipSuccess)
+            rc = fail(UTCNB_EHIP, std::string("padding: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        int keep = rc;
+        release(s);
+        delete s;
+        return keep < 0 ? keep : UTCNB_ENOMEM;
+    }
+    *out = s;
+    return UTCNB_OK;
+}
+
+int utcnb_destroy(utcnb_system* s)
+{
+    if (!s)
+        return UTCNB_OK;
+    (void)hipStreamSynchronize(s->stream);
+    release(s);
+    delete s;
+    return UTCNB_OK;
+}
+
+int utcnb_set_state(utcnb_system* s, const void* pos4, const void* vel4, int dtype, int where)
+{
+    if (!s || !pos4)
+        return fail(UTCNB_EINVAL, "utcnb_set_state: null handle or positions");
+    if ((dtype != UTCNB_F32 && dtype != UTCNB_F64) || (where != UTCNB_HOST && where != UTCNB_DEVICE))
+        return fail(UTCNB_EINVAL, "utcnb_set_state: bad dtype or location");
+    std::vector<double> vdefault;
+    int vdtype = dtype, vwhere = where;
+    if (!vel4) {
+        vdefault.assign(4 * (size_t)s->n, 0.0);
+        for (int i = 0; i < s->n; ++i)
+            vdefault[4 * (size_t)i + 3] = 1.0;
+        vel4 = vdefault.data();
+        vdtype = UTCNB_F64;
+        vwhere = UTCNB_HOST;
+    }
+    s->cur = 0;
+    int rc;
+    if (is_f32_state(s)) {
+        rc = upload4(s, pos4, dtype, where, s->p32[0]);
+        if (!rc)
+            rc = upload4(s, vel4, vdtype, vwhere, s->v32);
+    } else {
+        rc = upload4(s, pos4, dtype, where, s->p64[0]);
+        if (!rc)
+            rc = upload4(s, vel4, vdtype, vwhere, s->v64);
+        if (!rc && s->cfg.precision == UTCNB_PREC_MIXED)
+            UTCNB_HIP(launch_snapshot(s->p64[0], s->snap[0], s->n, s->stream));
+    }
+    if (rc)
+        return rc;
+    // the host default buffer must outlive the async copy
+    if (!vdefault.empty())
+        UTCNB_HIP(hipStreamSynchronize(s->stream));
+    s->has_state = true;
+    return UTCNB_OK;
+}
+
+int utcnb_get_state(utcnb_system* s, void* pos4, void* vel4, int dtype, int where)
+{
+    if (!s)
+        return fail(UTCNB_EINVAL, "utcnb_get_state: null handle");
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_get_state: no state set");
+    if ((dtype != UTCNB_F32 && dtype != UTCNB_F64) || (where != UTCNB_HOST && where != UTCNB_DEVICE))
+        return fail(UTCNB_EINVAL, "utcnb_get_state: bad dtype or location");
+    int rc = UTCNB_OK;
+    if (is_f32_state(s)) {
+        if (pos4)
+            rc = download4(s, s->p32[s->cur], pos4, dtype, where);
+        if (!rc && vel4) {
+            if (pos4 && where == UTCNB_HOST)
+                UTCNB_HIP(hipStreamSynchronize(s->stream));  // staging reuse
+            rc = download4(s, s->v32, vel4, dtype, where);
+        }
+    } else {
+        if (pos4)
+            rc = download4(s, cur_p64(s), pos4, dtype, where);
+        if (!rc && vel4) {
+            if (pos4)
+                UTCNB_HIP(hipStreamSynchronize(s->stream));
+            rc = download4(s, s->v64, vel4, dtype, where);
+        }
+    }
+    if (rc)
+        return rc;
+    if (where == UTCNB_HOST)
+        UTCNB_HIP(hipStreamSynchronize(s->stream));
+    return UTCNB_OK;
+}
+
+// One step on the current buffers (no bookkeeping of timing).
+static hipError_t step_once(utcnb_system* s)
+{
+    hipError_t e = hipSuccess;
+    const int c = s->cur;
+    switch (s->cfg.precision) {
+    case UTCNB_PREC_REFERENCE:
+        if (s->cfg.state_dtype == UTCNB_F32)
+            e = launch_ref_step_f32(s->p32[c], s->p32[1 - c], s->v32, s->n, (float)s->cfg.dt,
+                                    (float)s->cfg.eps2, (float)s->cfg.damping, 256, s->stream);
+        else
+            e = launch_ref_step_f64(s->p64[c], s->p64[1 - c], s->v64, s->n, s->cfg.dt,
+                                    s->cfg.eps2, s->cfg.damping, 256, s->stream);
+        s->cur = 1 - c;
+        break;
+    case UTCNB_PREC_FP32:
+        e = launch_fp32_step(s->args, s->p32[c], s->p32[1 - c], s->v32, s->stream);
+        s->cur = 1 - c;
+        break;
+    case UTCNB_PREC_MIXED:
+        e = launch_mixed_step(s->args, s->snap[c], s->snap[1 - c], s->p64[0], s->v64, s->stream);
+        s->cur = 1 - c;
+        break;
+    case UTCNB_PREC_FP64:
+        e = launch_fp64_step(s->args, s->p64[c], s->p64[1 - c], s->v64, s->stream);
+        s->cur = 1 - c;
+        break;
+    }
+    return e;
+}
+
+int utcnb_step(utcnb_system* s, int nsteps)
+{
+    if (!s || nsteps < 0)
+        return fail(UTCNB_EINVAL, "utcnb_step: null handle or negative step count");
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_step: no state set");
+    UTCNB_HIP(hipEventRecord(s->ev0, s->stream));
+    for (int k = 0; k < nsteps; ++k) {
+        hipError_t e = step_once(s);
+        if (e != hipSuccess)
+            return fail(UTCNB_EHIP, std::string("step kernel: ") + hipGetErrorString(e));
+    }
+    UTCNB_HIP(hipEventRecord(s->ev1, s->stream));
+    s->last_launches = nsteps;
+    s->timed = true;
+    return UTCNB_OK;
+}
+
+int utcnb_sync(utcnb_system* s)
+{
+    if (!s)
+        return fail(UTCNB_EINVAL, "utcnb_sync: null handle");
+    UTCNB_HIP(hipStreamSynchronize(s->stream));
+    return UTCNB_OK;
+}
+
+int utcnb_last_kernel_ms(utcnb_system* s, double* ms, int* launches)
+{
+    if (!s || !ms)
+        return fail(UTCNB_EINVAL, "utcnb_last_kernel_ms: null argument");
+    if (!s->timed)
+        return fail(UTCNB_ESTATE, "utcnb_last_kernel_ms: nothing timed yet");
+    UTCNB_HIP(hipEventSynchronize(s->ev1));
+    float f = 0.f;
+    UTCNB_HIP(hipEventElapsedTime(&f, s->ev0, s->ev1));
+    *ms = f;
+    if (launches)
+        *launches = s->last_launches;
+    return UTCNB_OK;
+}
+
+int utcnb_accel(utcnb_system* s, double* acc3)
+{
+    if (!s || !acc3)
+        return fail(UTCNB_EINVAL, "utcnb_accel: null argument");
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_accel: no state set");
+    hipError_t e = hipSuccess;
+    switch (s->cfg.precision) {
+    case UTCNB_PREC_REFERENCE:
+        if (s->cfg.state_dtype == UTCNB_F32)
+            e = launch_ref_accel_f32(s->p32[s->cur], s->n, (float)s->cfg.eps2, s->scratch,
+                                     s->stream);
+        else
+            e = launch_ref_accel_f64(s->p64[s->cur], s->n, s->cfg.eps2, s->scratch, s->stream);
+        break;
+    case UTCNB_PREC_FP32:
+        e = launch_fp32_accel(s->args, s->p32[s->cur], false, s->scratch, s->stream);
+        break;
+    case UTCNB_PREC_MIXED:
+        e = launch_fp32_accel(s->args, s->snap[s->cur], true, s->scratch, s->stream);
+        break;
+    case UTCNB_PREC_FP64:
+        e = launch_fp64_accel(s->args, s->p64[s->cur], s->scratch, s->stream);
+        break;
+    }
+    if (e != hipSuccess)
+        return fail(UTCNB_EHIP, std::string("field kernel: ") + hipGetErrorString(e));
+    UTCNB_HIP(hipMemcpyAsync(acc3, s->scratch, sizeof(double) * 3 * (size_t)s->n,
+                             hipMemcpyDeviceToHost, s->stream));
+    UTCNB_HIP(hipStreamSynchronize(s->stream));
+    return UTCNB_OK;
+}
+
+int utcnb_diagnostics(utcnb_system* s, double* kinetic, double* potential, double* momentum3)
+{
+    if (!s)
+        return fail(UTCNB_EINVAL, "utcnb_diagnostics: null handle");
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_diagnostics: no state set");
+    const double4* p;
+    const double4* v;
+    if (is_f32_state(s)) {
+        double4* tp = (double4*)s->stage;
+        double4* tv = tp + s->n;
+        UTCNB_HIP(launch_f32_to_f64(s->p32[s->cur], tp, s->n, s->stream));
+        UTCNB_HIP(launch_f32_to_f64(s->v32, tv, s->n, s->stream));
+        p = tp;
+        v = tv;
+    } else {
+        p = cur_p64(s);
+        v = s->v64;
+    }
+    UTCNB_HIP(launch_energy(p, v, s->n, s->cfg.eps2, s->scratch, s->stream));
+    double out[5];
+    UTCNB_HIP(hipMemcpyAsync(out, s->scratch, sizeof(out), hipMemcpyDeviceToHost, s->stream));
+    UTCNB_HIP(hipStreamSynchronize(s->stream));
+    if (kinetic)
+        *kinetic = out[0];
+    if (potential)
+        *potential = out[1];
+    if (momentum3) {
+        momentum3[0] = out[2];
+        momentum3[1] = out[3];
+        momentum3[2] = out[4];
+    }
+    return UTCNB_OK;
+}
+
+}  // extern "C"

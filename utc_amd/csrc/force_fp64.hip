@@ -1,0 +1,142 @@
+// force_fp64.hip -- FP64 all-pairs kernels: the on-device yardstick that the
+// FP32 and mixed paths are measured against at sizes a CPU cannot reach.
+//
+// Same mapping as the FP32 kernels (sources uniform per wave, scalar loads of
+// double4 bodies into SGPR pairs, S waves splitting the sources), with a
+// correctly rounded sqrt and division per pair and Kahan-compensated chunk
+// sums, so every field component carries only a few FP64 ulps of error.
+#include "internal.h"
+
+namespace utcnb {
+
+__device__ __forceinline__ void kahan_add64(double& s, double& c, double p)
+{
+    double y = p - c;
+    double t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+struct Epi64Step {
+    const double4* pos_old;
+    double4* pos_new;
+    double4* vel;
+    double dt, damping;
+    int kick;
+    __device__ __forceinline__ void operator()(int i, double ax, double ay, double az) const
+    {
+        double4 p = pos_old[i];
+        double4 v = vel[i];
+        double w = kick == UTCNB_KICK_REFERENCE ? p.w : 1.0;
+        v.x = (v.x + (ax * w) * dt) * damping;
+        v.y = (v.y + (ay * w) * dt) * damping;
+        v.z = (v.z + (az * w) * dt) * damping;
+        p.x += v.x * dt;
+        p.y += v.y * dt;
+        p.z += v.z * dt;
+        pos_new[i] = p;
+        vel[i] = v;
+    }
+};
+
+struct Epi64Accel {
+    double* acc3;
+    __device__ __forceinline__ void operator()(int i, double ax, double ay, double az) const
+    {
+        acc3[3 * (size_t)i + 0] = ax;
+        acc3[3 * (size_t)i + 1] = ay;
+        acc3[3 * (size_t)i + 2] = az;
+    }
+};
+
+template <int S, int W, class Epi>
+__global__ __launch_bounds__(64 * W) void force64_kernel(const double4* __restrict__ src, int n,
+                                                         int n_pad, double eps2, Epi epi)
+{
+    constexpr int T = W / S;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int tile = blockIdx.x * T + wave / S;
+    const int part = wave % S;
+    const int i = tile * 64 + lane;
+    const double4 pi = src[i < n ? i : n - 1];
+    double hx = 0, hy = 0, hz = 0, cx = 0, cy = 0, cz = 0;
+    const int len = n_pad / S;
+    const int j0 = part * len;
+    for (int jc = j0; jc < j0 + len; jc += kChunk) {
+        double ax = 0, ay = 0, az = 0;
+#pragma unroll 4
+        for (int j = jc; j < jc + kChunk; ++j) {
+            const double4 b = src[j];
+            double rx = b.x - pi.x, ry = b.y - pi.y, rz = b.z - pi.z;
+            double d2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, __builtin_fma(rz, rz, eps2)));
+            double inv = 1.0 / __builtin_sqrt(d2);
+            double s = b.w * (inv * inv * inv);
+            ax = __builtin_fma(rx, s, ax);
+            ay = __builtin_fma(ry, s, ay);
+            az = __builtin_fma(rz, s, az);
+        }
+        kahan_add64(hx, cx, ax);
+        kahan_add64(hy, cy, ay);
+        kahan_add64(hz, cz, az);
+    }
+    if constexpr (S > 1) {
+        __shared__ double red[W][6][64];
+        red[wave][0][lane] = hx;
+        red[wave][1][lane] = hy;
+        red[wave][2][lane] = hz;
+        red[wave][3][lane] = cx;
+        red[wave][4][lane] = cy;
+        red[wave][5][lane] = cz;
+        __syncthreads();
+        if (part != 0)
+            return;
+        for (int q = 1; q < S; ++q) {
+            kahan_add64(hx, cx, red[wave + q][0][lane] - red[wave + q][3][lane]);
+            kahan_add64(hy, cy, red[wave + q][1][lane] - red[wave + q][4][lane]);
+            kahan_add64(hz, cz, red[wave + q][2][lane] - red[wave + q][5][lane]);
+        }
+    }
+    if (i < n)
+        epi(i, hx - cx, hy - cy, hz - cz);
+}
+
+template <int S, class Epi>
+static hipError_t launch64_tile(const double4* src, int n, int n_pad, double eps2, const Epi& epi,
+                                hipStream_t st)
+{
+    constexpr int W = S < 4 ? 4 : S;
+    constexpr int T = W / S;
+    const int tiles = (n + 63) / 64;
+    const int blocks = (tiles + T - 1) / T;
+    hipLaunchKernelGGL((force64_kernel<S, W, Epi>), dim3(blocks), dim3(64 * W), 0, st, src, n,
+                       n_pad, eps2, epi);
+    return hipGetLastError();
+}
+
+template <class Epi>
+static hipError_t launch64(const double4* src, int n, int n_pad, double eps2, const Epi& epi,
+                           hipStream_t st)
+{
+    const int waves = (n + 63) / 64;
+    if (waves >= 2048)
+        return launch64_tile<1>(src, n, n_pad, eps2, epi, st);
+    if (waves * 4 >= 2048)
+        return launch64_tile<4>(src, n, n_pad, eps2, epi, st);
+    return launch64_tile<16>(src, n, n_pad, eps2, epi, st);
+}
+
+hipError_t launch_fp64_step(const StepArgs& a, const double4* pos_old, double4* pos_new,
+                            double4* vel, hipStream_t st)
+{
+    Epi64Step epi{pos_old, pos_new, vel, a.dt, a.damping, a.kick};
+    return launch64(pos_old, a.n, a.n_pad, a.eps2, epi, st);
+}
+
+hipError_t launch_fp64_accel(const StepArgs& a, const double4* src, double* acc3, hipStream_t st)
+{
+    Epi64Accel epi{acc3};
+    return launch64(src, a.n, a.n_pad, a.eps2, epi, st);
+}
+
+}  // namespace utcnb

@@ -1,0 +1,100 @@
+// internal.h -- shared declarations of the gfx950 nbody library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "utcnb.h"
+
+namespace utcnb {
+
+// ---- error plumbing ------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+#define UTCNB_HIP(call)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::utcnb::fail(UTCNB_EHIP, std::string(#call) + ": " +                 \
+                                                 hipGetErrorString(e_));                 \
+    } while (0)
+
+// ---- layout constants ------------------------------------------------------
+// Source (j) arrays of the fast kernels are padded to a multiple of
+// kPadJ bodies with zero-mass bodies placed far away, so every wave streams a
+// whole number of 64-body chunks and every j-split is chunk aligned.
+constexpr int kChunk = 64;           // j per compensated partial sum
+constexpr int kPadJ = 1024;          // = kChunk * largest j-split (16 waves)
+constexpr float kPadCoord = 1.0e18f; // position of padding bodies
+inline int64_t pad_j(int64_t n) { return (n + kPadJ - 1) / kPadJ * kPadJ; }
+
+// ---- force-kernel tiles ---------------------------------------------------
+// A tile is (K bodies i per lane, S waves splitting the j range per workgroup).
+struct Tile {
+    int k;
+    int s;
+    const char* name;
+};
+int tile_count();
+const Tile& tile_get(int id);
+// Heuristic default: enough waves to cover the chip (256 CUs x 8 waves).
+int tile_default(int64_t n, int cu_count);
+
+// ---- launchers (force.hip) ------------------------------------------------
+struct StepArgs {
+    int n;          // bodies
+    int n_pad;      // padded j length (fast kernels)
+    float eps2f;
+    double eps2;
+    double dt, damping;
+    int kick;       // UTCNB_KICK_*
+    bool newton;
+    int tile;       // tile id
+};
+
+// Reference arithmetic, one fused step (pos_new/vel written), T = float/double.
+hipError_t launch_ref_step_f32(const float4* pos_old, float4* pos_new, float4* vel, int n,
+                               float dt, float eps2, float damping, int block,
+                               hipStream_t st);
+hipError_t launch_ref_step_f64(const double4* pos_old, double4* pos_new, double4* vel, int n,
+                               double dt, double eps2, double damping, int block,
+                               hipStream_t st);
+
+// Plain FP32: forces from pos_old (padded float4), integrate into pos_new / vel.
+hipError_t launch_fp32_step(const StepArgs& a, const float4* pos_old, float4* pos_new,
+                            float4* vel, hipStream_t st);
+// Mixed: forces from snap_old (padded float4), FP64 state updated in place,
+// new snapshot written to snap_new.
+hipError_t launch_mixed_step(const StepArgs& a, const float4* snap_old, float4* snap_new,
+                             double4* pos, double4* vel, hipStream_t st);
+// FP64: forces from pos_old (padded double4) into pos_new / vel.
+hipError_t launch_fp64_step(const StepArgs& a, const double4* pos_old, double4* pos_new,
+                            double4* vel, hipStream_t st);
+// Accelerations only (FP64 out [3n]) for each arithmetic.
+hipError_t launch_fp32_accel(const StepArgs& a, const float4* src, bool compensated,
+                             double* acc3, hipStream_t st);
+hipError_t launch_fp64_accel(const StepArgs& a, const double4* src, double* acc3,
+                             hipStream_t st);
+hipError_t launch_ref_accel_f32(const float4* src, int n, float eps2, double* acc3,
+                                hipStream_t st);
+hipError_t launch_ref_accel_f64(const double4* src, int n, double eps2, double* acc3,
+                                hipStream_t st);
+
+// ---- helpers (aux.hip) ----------------------------------------------------
+// Fill padding bodies [n, n_pad) of a float4 / double4 source array.
+hipError_t launch_pad_f32(float4* p, int n, int n_pad, hipStream_t st);
+hipError_t launch_pad_f64(double4* p, int n, int n_pad, hipStream_t st);
+// FP64 state -> FP32 snapshot (x,y,z,m rounded to nearest).
+hipError_t launch_snapshot(const double4* pos, float4* snap, int n, hipStream_t st);
+// Type conversions between state arrays.
+hipError_t launch_f32_to_f64(const float4* src, double4* dst, int n, hipStream_t st);
+hipError_t launch_f64_to_f32(const double4* src, float4* dst, int n, hipStream_t st);
+// FP64 diagnostics: kinetic, potential (pairs i<j) and momentum, accumulated
+// into out[5] = {K, W, Px, Py, Pz} (device, zeroed by the launcher).
+hipError_t launch_energy(const double4* pos, const double4* vel, int n, double eps2,
+                         double* out5, hipStream_t st);
+
+}  // namespace utcnb
