@@ -1,0 +1,284 @@
+#!/usr/bin/env python3
+"""Benchmark of the nbody hot path on one MI355X.
+
+Workload (default): N = 262144 bodies, Plummer sphere (a = 1, M = 1, equal
+masses, seed 0), eps = 0.002, dt = 2.5e-4, physical kick, K timed steps.  A
+step is one launch of the fused all-pairs force + integrator kernel; the timed
+region covers exactly K steps with the state resident in HBM.
+
+Prints ONE JSON line (stdout).  `value` is interactions/s of the mixed path
+(FP32 pairs with compensated sums, FP64 state): N^2 * K / wall time.  Also
+reported: the plain FP32 path, the Newton-step variant, accuracy against the
+on-device FP64 path (acceleration and final-position relative error), energy
+drift, forward+adjoint throughput, the roofline of the force kernel and a
+CPU baseline (the oracle's restatement of the reference's sequential CPU path,
+threaded over bodies with the reference's partition rule, on a row sample).
+
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--n 262144]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "interactions/s fwd & fwd+adjoint at N=262144 vs plain FP32 GPU; rel err"
+PEAK_FP32 = 157.3e12      # MI355X FP32 vector peak (AMD spec), FLOP/s
+FLOP_PER_PAIR = 20        # reference convention, benchapps/nbody/seq/bodysystem.h:75-92
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def relnorm(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def timed_steps(U, prec, n, c, steps, warmup, flags=0, tile=0):
+    """Run warmup + timed steps; returns (wall_ms, kernel_ms, launches, system)."""
+    s = U.System(n, prec, c["eps"] ** 2, c["dt"], flags=flags, tile=tile)
+    s.set_state(c["pos"], c["vel"])
+    if warmup:
+        s.step(warmup)
+    s.sync()
+    # reset to the initial state so the timed steps are the configuration's steps
+    s.set_state(c["pos"], c["vel"])
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    wall = (time.perf_counter() - t0) * 1e3
+    kms, launches = s.last_kernel_ms()
+    return wall, kms, launches, s
+
+
+def read_traffic(path: Path):
+    """HBM bytes per launch of the force kernel from the committed PMC summary."""
+    if not path.exists():
+        return None
+    txt = path.read_text()
+    m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(c, n, threads, rows):
+    """The oracle's restatement of benchapps/nbody/seq/bodysystem.cc:103-133
+    (bit-identical to the reference), rows [0, rows) x all n sources, one
+    force evaluation, threaded over rows with the reference's partition rule."""
+    from oracle import oracle as O
+    pos32 = c["pos"].astype(np.float32)
+    eps2 = np.float32(c["eps"]) * np.float32(c["eps"])
+    O.gravitation_rows_f32(pos32, eps2, 0, min(256, rows), threads)  # warm the pages
+    t0 = time.perf_counter()
+    O.gravitation_rows_f32(pos32, eps2, 0, rows, threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": rows * float(n) / dt,
+        "unit": "interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one force evaluation of rows 0..{rows - 1} against all {n} sources "
+                  f"(FP32, reference order) = {rows * float(n):.3g} pair interactions, "
+                  f"{dt:.2f} s wall on {threads} threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=8192)
+    ap.add_argument("--only", choices=["mixed", "fp32"], default=None,
+                    help="time one path only and print its line (for profiler runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a plain 256 MiB device copy (to scale HBM counters) and exit")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        nbytes = 256 << 20
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 3}), flush=True)
+        return
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "path is design-only in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import utc_amd as U
+    from tests.nbcases import plummer
+
+    n, K, W = args.n, args.steps, args.warmup
+    c = plummer(n, eps=0.002, dt=2.5e-4, steps=K)
+    name, cus, clk = U.device_info()
+    log(f"device {name} CUs={cus} clock={clk} kHz; n={n} steps={K} warmup={W}")
+
+    pairs = float(n) * float(n)
+    results = {}
+    systems = {}
+    variants = (("mixed", U.PREC_MIXED, U.FLAG_NO_NEWTON),
+                ("mixed_newton", U.PREC_MIXED, 0),
+                ("fp32", U.PREC_FP32, U.FLAG_NO_NEWTON),
+                ("fp32_newton", U.PREC_FP32, 0))
+    if args.only:
+        variants = [v for v in variants if v[0] == args.only]
+    for label, prec, flags in variants:
+        wall, kms, launches, s = timed_steps(U, prec, n, c, K, W, flags=flags)
+        results[label] = dict(wall_ms=wall, kernel_ms=kms, launches=launches,
+                              interactions_per_s=pairs * K / (wall * 1e-3),
+                              kernel_interactions_per_s=pairs * launches / (kms * 1e-3))
+        systems[label] = s
+        log(f"{label}: {wall / K:.3f} ms/step wall, {kms / launches:.3f} ms/step kernel, "
+            f"{results[label]['interactions_per_s']:.4e} interactions/s")
+
+    if args.only:
+        r = results[args.only]
+        print(json.dumps({"metric": METRIC, "value": r["interactions_per_s"], "unit": "interactions/s",
+                          "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": r["wall_ms"] / K,
+                          "only": args.only, "kernel_ms_per_step": r["kernel_ms"] / r["launches"]}),
+              flush=True)
+        return
+
+    # ---- accuracy against the on-device FP64 path --------------------------------
+    s64 = U.System(n, U.PREC_FP64, c["eps"] ** 2, c["dt"])
+    s64.set_state(c["pos"], c["vel"])
+    a64 = s64.accel()
+    k0, w0, _ = s64.diagnostics()
+    s64.step(K)
+    p64, _ = s64.get_state()
+    acc = {}
+    for label, prec, flags in (("mixed", U.PREC_MIXED, U.FLAG_NO_NEWTON),
+                               ("mixed_newton", U.PREC_MIXED, 0),
+                               ("fp32", U.PREC_FP32, U.FLAG_NO_NEWTON),
+                               ("fp32_newton", U.PREC_FP32, 0)):
+        s = systems[label]
+        p, _ = s.get_state()
+        s.set_state(c["pos"], c["vel"])
+        a = s.accel()
+        acc[label] = dict(accel_relerr=relnorm(a, a64),
+                          final_pos_relerr=relnorm(p[:, :3], p64[:, :3]),
+                          final_pos_maxabs=float(np.abs(p[:, :3] - p64[:, :3]).max()))
+        log(f"{label}: accel rel err {acc[label]['accel_relerr']:.3e}, final-position rel err "
+            f"{acc[label]['final_pos_relerr']:.3e}")
+    # energy drift of the mixed path over the K steps
+    sm = systems["mixed"]
+    sm.set_state(c["pos"], c["vel"])
+    e0k, e0w, _ = sm.diagnostics()
+    sm.step(K)
+    e1k, e1w, p1 = sm.diagnostics()
+    e0, e1 = e0k + e0w, e1k + e1w
+    energy = dict(E0=e0, E1=e1, rel_drift=abs(e1 - e0) / abs(e0), virial_2K_over_W=2 * e0k / abs(e0w),
+                  momentum_end=[float(x) for x in p1])
+    log(f"energy drift {energy['rel_drift']:.3e}, virial 2K/|W| {energy['virial_2K_over_W']:.4f}")
+
+    # ---- forward + adjoint ----------------------------------------------------------
+    fwd_adj = None
+    if not args.no_adjoint and hasattr(U.lib(), "utcnb_adjoint"):
+        try:
+            tgt = U.generate(U.GEN_TARGETS, n, 0x5EED7A56E7, 0.5)
+            sa = U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON)
+            sa.set_state(c["pos"], c["vel"])
+            sa.adjoint(2, 1, tgt)
+            sa.set_state(c["pos"], c["vel"])
+            sa.sync()
+            t0 = time.perf_counter()
+            loss, dx, dv, dm = sa.adjoint(K, 10, tgt)
+            wall = (time.perf_counter() - t0) * 1e3
+            kms, launches = sa.last_kernel_ms()
+            fwd_rate = results["mixed"]["interactions_per_s"]
+            fe = pairs * K / (wall * 1e-3)
+            fwd_adj = dict(wall_ms=wall, ckpt_every=10, loss=loss,
+                           forward_equivalent_interactions_per_s=fe,
+                           ratio_vs_forward=fe / fwd_rate,
+                           pair_evaluations=launches,
+                           total_pair_evaluations_per_s=pairs * launches / (wall * 1e-3),
+                           total_ratio_vs_forward=pairs * launches / (wall * 1e-3) / fwd_rate)
+            log(f"fwd+adjoint: {wall:.1f} ms, forward-equivalent {fe:.4e}/s "
+                f"({fe / fwd_rate:.3f}x forward), total pair evaluations {launches}")
+        except Exception as e:  # reported, never hidden
+            fwd_adj = {"error": str(e)}
+            log(f"fwd+adjoint failed: {e}")
+
+    # ---- roofline of the force kernel ------------------------------------------
+    m = results["mixed"]
+    t_launch = m["kernel_ms"] / m["launches"] * 1e-3
+    achieved = FLOP_PER_PAIR * pairs / t_launch
+    traffic = read_traffic(ROOT / "profiles" / "force_mixed_n262144_pmc.txt")
+    roofline = {
+        "bound": "valu",
+        "achieved": achieved / 1e12,
+        "peak": PEAK_FP32 / 1e12,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP32,
+        "traffic": traffic,
+        "kernel": "force_kernel<K,S,COMP=1> + EpiMixed (fused force, FP64 kick-drift, snapshot)",
+        "flop_per_launch": FLOP_PER_PAIR * pairs,
+        "avg_launch_ms": t_launch * 1e3,
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("BENCH_CPU_THREADS", min(16, os.cpu_count() or 1)))
+        cpu = cpu_baseline(c, n, threads, args.cpu_rows)
+        log(f"cpu baseline: {cpu['value']:.4e} interactions/s on {threads} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": m["interactions_per_s"],
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": m["wall_ms"] / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded Plummer sphere)",
+        "config": {
+            "workload": f"nbody all-pairs force + kick-drift, N={n} Plummer (a=1, M=1), eps=0.002, "
+                        f"dt=2.5e-4, {K} steps, FP32 compensated pairs + FP64 state",
+            "n": n, "eps": 0.002, "dt": 2.5e-4, "steps": K, "precision": "mixed",
+            "parallelism": "single GPU",
+        },
+        "vs_plain_fp32": {
+            "fp32_interactions_per_s": results["fp32"]["interactions_per_s"],
+            "mixed_over_fp32": m["interactions_per_s"] / results["fp32"]["interactions_per_s"],
+        },
+        "variants": results,
+        "accuracy_vs_fp64": acc,
+        "energy": energy,
+        "fwd_adjoint": fwd_adj,
+        "fraction_of_fp32_peak": achieved / PEAK_FP32,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "device": {"name": name, "cu": cus, "clock_khz": clk},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -167,8 +167,13 @@ int utcnb_batch_set_state(utcnb_batch* b, const double* pos4, const double* vel4
 int utcnb_batch_get_state(utcnb_batch* b, double* pos4, double* vel4);
 /*
  * mode 0: one captured HIP graph of persistent step kernels over all systems
- *         (replayed per call; captured on first use);
- * mode 1: per-system launches (the baseline).
+ *         (captured on first use for a given nsteps/ckpt_every/adjoint, then
+ *         replayed; every run starts from the FP64 state);
+ * mode 1: per-system launches of the single-system kernels (the baseline);
+ * mode 2: the persistent kernels of mode 0 launched eagerly (no graph).
+ * cfg->tile of the batch: 0 = pick the waves per work item (4, 8, 16) by
+ * timing on the first mode 0/2 run, 1..3 = fixed 4, 8 or 16.
+ * The batched adjoint supports the physical kick only.
  * When target3 is non-NULL the adjoint of the same loss as utcnb_adjoint runs
  * after the forward steps (checkpoint every ckpt_every) and losses[nsys],
  * dx0/dv0 [3 * sum(sizes)], dm [sum(sizes)] receive per-system results.

@@ -398,7 +398,9 @@ DEFINE_WRITE_FRAMES(orc_write_frames_f64, double)
  * analog is BodySystem<double> (seq/bodysystem.cc:184-185), whose plain
  * summation is restated above.  Here every per-body sum is compensated
  * (Neumaier), so the result is accurate to a few FP64 ulps of the largest
- * term and serves as the error yardstick for the FP32 and mixed paths.     */
+ * term and serves as the error yardstick for the FP32 and mixed paths.
+ * The self pair is skipped: for eps2 > 0 it contributes exactly zero (r = 0),
+ * and skipping it keeps the functions finite at eps2 = 0.                   */
 typedef struct {
     double s, c;
 } ksum;
@@ -421,6 +423,8 @@ void orc_accel_f64(int n, const double* pos4, double eps2, double* acc3)
         const double* pi = pos4 + 4 * (size_t)i;
         ksum ax = {0, 0}, ay = {0, 0}, az = {0, 0};
         for (int j = 0; j < n; ++j) {
+            if (j == i)
+                continue;
             const double* pj = pos4 + 4 * (size_t)j;
             double rx = pj[0] - pi[0], ry = pj[1] - pi[1], rz = pj[2] - pi[2];
             double d2 = rx * rx + ry * ry + rz * rz + eps2;
@@ -442,6 +446,8 @@ void orc_accel_ld(int n, const double* pos4, double eps2, double* acc3)
         const double* pi = pos4 + 4 * (size_t)i;
         long double ax = 0, ay = 0, az = 0;
         for (int j = 0; j < n; ++j) {
+            if (j == i)
+                continue;
             const double* pj = pos4 + 4 * (size_t)j;
             long double rx = (long double)pj[0] - pi[0];
             long double ry = (long double)pj[1] - pi[1];
@@ -583,6 +589,8 @@ double orc_adjoint_f64(int n, int steps, const double* pos4, const double* vel4,
             const double* ai = al + 3 * (size_t)i;
             ksum gx = {0, 0}, gy = {0, 0}, gz = {0, 0}, gm = {0, 0};
             for (int j = 0; j < n; ++j) {
+                if (j == i)
+                    continue;
                 const double* pj = x + 4 * (size_t)j;
                 const double* aj = al + 3 * (size_t)j;
                 double rx = pj[0] - pi[0], ry = pj[1] - pi[1], rz = pj[2] - pi[2];

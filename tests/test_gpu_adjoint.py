@@ -1,0 +1,183 @@
+"""GPU parity of the adjoint (reverse mode through force + kick-drift).
+
+Checks: the FP64 GPU adjoint against the CPU FP64 reverse mode of the oracle
+(itself pinned by finite differences in test_oracle.py); GPU gradients against
+FP64 central finite differences at the config-4 size (N=16384, 50 steps,
+checkpoint every 10, 64 random coordinates); the mixed (FP32-pair) adjoint
+against the FP64 one; checkpoint-interval invariance; and a torch.autograd
+wrapper, which exists only here in the test harness.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import utc_amd as U
+from nbcases import config4, relnorm
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_case(n=256, steps=12, seed=3, kick=U.KICK_PHYSICAL):
+    pos, vel = U.generate(U.GEN_CUBE_LOGM, n, seed, 0.05, 0.5, 2.0)
+    tgt = U.generate(U.GEN_TARGETS, n, seed + 1, 0.5)
+    return pos, vel, tgt
+
+
+@pytest.mark.parametrize("kick,damping", [(U.KICK_PHYSICAL, 1.0), (U.KICK_REFERENCE, 0.995)])
+def test_fp64_adjoint_matches_cpu_reverse_mode(kick, damping):
+    n, steps, eps2, dt = 256, 12, 1e-3, 2e-3
+    pos, vel, tgt = small_case(n, steps)
+    say(f"case: FP64 adjoint vs CPU oracle kick={kick} n={n}")
+    L0, dx0, dv0, dm0 = O.adjoint_f64(pos, vel, tgt, steps, eps2, dt, damping, kick)
+    with U.System(n, U.PREC_FP64, eps2, dt, damping=damping, kick=kick) as s:
+        s.set_state(pos, vel)
+        L, dx, dv, dm = s.adjoint(steps, 5, tgt)
+    assert L == pytest.approx(L0, rel=1e-12)
+    assert relnorm(dx, dx0) <= 1e-11
+    assert relnorm(dv, dv0) <= 1e-11
+    assert relnorm(dm, dm0) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", [U.PREC_MIXED, U.PREC_FP64])
+def test_checkpoint_interval_does_not_change_gradients(prec):
+    n, steps, eps2, dt = 2048, 20, 1e-4, 1e-3
+    pos, vel, tgt = small_case(n, steps)
+    say(f"case: checkpoint invariance prec={prec}")
+    outs = []
+    with U.System(n, prec, eps2, dt) as s:
+        for k in (1, 7, 20):
+            s.set_state(pos, vel)
+            outs.append(s.adjoint(steps, k, tgt))
+    for L, dx, dv, dm in outs[1:]:
+        assert L == outs[0][0]
+        assert np.array_equal(dx, outs[0][1])
+        assert np.array_equal(dv, outs[0][2])
+        assert np.array_equal(dm, outs[0][3])
+
+
+def fp64_final(s, pos, vel, steps):
+    s.set_state(pos, vel)
+    s.step(steps)
+    p, _ = s.get_state()
+    return p[:, :3]
+
+
+def central_difference(p_plus, p_minus, tgt, h):
+    """(L(+h) - L(-h)) / 2h for L = (1/n) sum |x - t|^2, evaluated per body as
+    (x+ - x-) . (x+ + x- - 2t): the same number without subtracting two
+    nearly equal totals over all n bodies."""
+    d = (p_plus - p_minus) * (p_plus + p_minus - 2.0 * tgt)
+    return float(d.sum() / (len(tgt) * 2.0 * h))
+
+
+@pytest.fixture(scope="module")
+def config4_grads():
+    c = config4()
+    n, steps = len(c["pos"]), c["steps"]
+    eps2 = c["eps"] ** 2
+    say(f"case: config4 adjoint n={n} steps={steps}")
+    res = {}
+    for prec in (U.PREC_FP64, U.PREC_MIXED):
+        with U.System(n, prec, eps2, c["dt"], kick=c["kick"]) as s:
+            s.set_state(c["pos"], c["vel"])
+            res[prec] = s.adjoint(steps, c["ckpt"], c["target"])
+    return c, res
+
+
+def test_config4_gradients_match_fp64_central_differences(config4_grads):
+    """64 random coordinates (positions, velocities, masses) of config 4; the
+    loss of each perturbed input is an FP64 forward on the GPU (itself checked
+    against the CPU FP64 oracle in test_gpu_forward.py).  FP64 rounding leaves
+    ~1e-13 of noise in the per-body sum of position differences whatever the
+    step, so the steps are as large as the loss's curvature allows: 1e-6 for a
+    position (close pairs sit at eps = 0.01), 1e-4 for a velocity and 5% of the
+    body's mass for a mass; the loss is close to quadratic in v and m over 50
+    steps, where central differences are exact.  Tolerance: 1e-5 relative per
+    coordinate."""
+    c, res = config4_grads
+    n, steps = len(c["pos"]), c["steps"]
+    eps2 = c["eps"] ** 2
+    rng = np.random.default_rng(11)
+    kinds = rng.integers(0, 3, 64)       # 0: position, 1: velocity, 2: mass
+    bodies = rng.integers(0, n, 64)
+    comps = rng.integers(0, 3, 64)
+    fds, g64, gmx = [], [], []
+    with U.System(n, U.PREC_FP64, eps2, c["dt"], kick=c["kick"]) as s:
+        for kind, i, comp in zip(kinds, bodies, comps):
+            if kind == 2:
+                comp = 3
+            arr = c["vel"] if kind == 1 else c["pos"]
+            h = (1e-6, 1e-4, 0.05 * c["pos"][i, 3])[kind]
+            a = arr.copy(); a[i, comp] += h
+            b = arr.copy(); b[i, comp] -= h
+            if kind == 1:
+                pp = fp64_final(s, c["pos"], a, steps)
+                pm = fp64_final(s, c["pos"], b, steps)
+            else:
+                pp = fp64_final(s, a, c["vel"], steps)
+                pm = fp64_final(s, b, c["vel"], steps)
+            fds.append(central_difference(pp, pm, c["target"], h))
+            for store, prec in ((g64, U.PREC_FP64), (gmx, U.PREC_MIXED)):
+                L, dx, dv, dm = res[prec]
+                store.append(dm[i] if kind == 2 else (dv if kind == 1 else dx)[i, comp])
+    fds, g64, gmx = map(np.array, (fds, g64, gmx))
+    e64 = np.abs(g64 - fds) / np.abs(fds)
+    emx = np.abs(gmx - fds) / np.abs(fds)
+    say(f"FD check: FP64 max rel {e64.max():.3e}, mixed max rel {emx.max():.3e}, "
+        f"mixed normwise {relnorm(gmx, fds):.3e}")
+    assert e64.max() <= 1e-5
+    assert emx.max() <= 1e-5
+
+
+def test_config4_mixed_adjoint_close_to_fp64(config4_grads):
+    c, res = config4_grads
+    L64, dx64, dv64, dm64 = res[U.PREC_FP64]
+    L, dx, dv, dm = res[U.PREC_MIXED]
+    assert L == pytest.approx(L64, rel=1e-9)
+    assert relnorm(dx, dx64) <= 1e-5
+    assert relnorm(dv, dv64) <= 1e-5
+    assert relnorm(dm, dm64) <= 1e-5
+
+
+def test_autograd_function_wrapper():
+    """A torch.autograd.Function whose forward and backward run through the C
+    ABI (utcnb_adjoint); it lives only in this test harness."""
+    import torch
+
+    class NBodyLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x0, v0, m, target, eps2, dt, steps, ckpt):
+            n = x0.shape[0]
+            pos = np.concatenate([x0.detach().cpu().numpy(), m.detach().cpu().numpy()[:, None]], 1)
+            vel = np.concatenate([v0.detach().cpu().numpy(), np.ones((n, 1))], 1)
+            with U.System(n, U.PREC_MIXED, eps2, dt) as s:
+                s.set_state(pos, vel)
+                L, dx, dv, dm = s.adjoint(steps, ckpt, target.detach().cpu().numpy())
+            ctx.save_for_backward(torch.from_numpy(dx), torch.from_numpy(dv), torch.from_numpy(dm))
+            return torch.tensor(L, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gL):
+            dx, dv, dm = ctx.saved_tensors
+            return gL * dx, gL * dv, gL * dm, None, None, None, None, None
+
+    n, steps, eps2, dt = 512, 10, 1e-3, 1e-3
+    pos, vel, tgt = small_case(n, steps)
+    x0 = torch.tensor(pos[:, :3], requires_grad=True)
+    v0 = torch.tensor(vel[:, :3], requires_grad=True)
+    m = torch.tensor(pos[:, 3], requires_grad=True)
+    say("case: autograd wrapper")
+    L = NBodyLoss.apply(x0, v0, m, torch.tensor(tgt), eps2, dt, steps, 5)
+    (2.0 * L).backward()
+    L0, dx0, dv0, dm0 = O.adjoint_f64(pos, vel, tgt, steps, eps2, dt, 1.0, 1)
+    assert float(L) == pytest.approx(L0, rel=1e-9)
+    assert relnorm(x0.grad.numpy(), 2 * dx0) <= 1e-5
+    assert relnorm(v0.grad.numpy(), 2 * dv0) <= 1e-5
+    assert relnorm(m.grad.numpy(), 2 * dm0) <= 1e-5

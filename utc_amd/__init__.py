@@ -222,3 +222,58 @@ def device_info():
 def tiles():
     L = lib()
     return [L.utcnb_tile_name(t).decode() for t in range(1, L.utcnb_tile_count() + 1)]
+
+
+class Batch:
+    """Many independent systems in one device allocation (utcnb_batch_*)."""
+
+    def __init__(self, sizes, eps2, dt, damping=1.0, kick=KICK_PHYSICAL, tile=0, stream=None):
+        self.sizes = [int(x) for x in sizes]
+        self.total = sum(self.sizes)
+        self.cfg = Config(0, PREC_MIXED, kick, F64, 0, tile, eps2, damping, dt)
+        arr = (C.c_int32 * len(self.sizes))(*self.sizes)
+        self.h = C.c_void_p()
+        check(lib().utcnb_batch_create(len(self.sizes), arr, C.byref(self.cfg), stream, C.byref(self.h)),
+              "utcnb_batch_create")
+
+    def close(self):
+        if self.h:
+            lib().utcnb_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, pos4, vel4):
+        p = np.ascontiguousarray(pos4, np.float64)
+        v = np.ascontiguousarray(vel4, np.float64)
+        check(lib().utcnb_batch_set_state(self.h, ptr(p), ptr(v)), "utcnb_batch_set_state")
+
+    def get_state(self):
+        p = np.zeros((self.total, 4))
+        v = np.zeros((self.total, 4))
+        check(lib().utcnb_batch_get_state(self.h, ptr(p), ptr(v)), "utcnb_batch_get_state")
+        return p, v
+
+    def run(self, nsteps, mode=0, ckpt_every=0, target3=None):
+        """Returns ms, or (ms, losses, dx, dv, dm) when target3 is given."""
+        ms = C.c_double()
+        if target3 is None:
+            check(lib().utcnb_batch_run(self.h, nsteps, mode, 0, None, None, None, None, None, C.byref(ms)),
+                  "utcnb_batch_run")
+            return ms.value
+        t = np.ascontiguousarray(target3, np.float64)
+        losses = np.zeros(len(self.sizes))
+        dx = np.zeros((self.total, 3)); dv = np.zeros((self.total, 3)); dm = np.zeros(self.total)
+        check(lib().utcnb_batch_run(self.h, nsteps, mode, ckpt_every, ptr(t), ptr(losses), ptr(dx), ptr(dv),
+                                    ptr(dm), C.byref(ms)), "utcnb_batch_run")
+        return ms.value, losses, dx, dv, dm

@@ -37,6 +37,9 @@ HIP_FLAGS = [
     "-fPIC",
     "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # 64-byte aligned loop headers: the source loop of the force kernels runs
+    # ~25% slower on MI355X when its header straddles a 32-byte fetch block.
+    "-falign-loops=64",
     "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}",
     f"-I{CSRC}",
@@ -52,6 +55,7 @@ LIB_SOURCES = [
     "aux.hip",
     "adjoint.hip",
     "batch.hip",
+    "batch_api.cpp",
     "capi.cpp",
     "gen.cpp",
     "autotune.cpp",

@@ -1,0 +1,611 @@
+// batch_api.cpp -- the batched-systems part of the C ABI (utcnb_batch_*).
+//
+// The reference's analog is its multi-task driver: T independent
+// Task<BodySystemSGPU> objects, one worker thread and stream each
+// (benchapps/nbody/utc/nbody_mtask.cc:185-231).  Here the systems share one
+// concatenated device state and one persistent launch per step (batch.hip);
+// the sequence of launches of a run is captured once in a HIP graph and
+// replayed.  Mode 1 keeps the per-system structure (one launch of the
+// single-system kernels per system and step) as the baseline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <new>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+using namespace utcnb;
+
+namespace {
+
+constexpr int kWs[3] = {4, 8, 16};  // waves per work item (batch tiles 1..3)
+
+template <typename T>
+int dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0)
+        return UTCNB_OK;
+    if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return fail(UTCNB_ENOMEM, "utcnb_batch: hipMalloc failed");
+    }
+    return UTCNB_OK;
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p)
+        (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct utcnb_batch {
+    utcnb_config cfg{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int cu_count = 256;
+    int nsys = 0;
+    std::vector<int> n, npad, off;
+    long long total = 0, total_real = 0;
+    double4* pos = nullptr;
+    double4* vel = nullptr;
+    float4* snap[2] = {nullptr, nullptr};
+    int* d_off = nullptr;
+    int* d_n = nullptr;
+    int* d_chunks = nullptr;
+    int2* d_items[3] = {nullptr, nullptr, nullptr};
+    int* d_start[3] = {nullptr, nullptr, nullptr};
+    int nblocks[3] = {0, 0, 0};
+    int wsel = -1;  // index into kWs, -1 = not tuned yet
+    bool has_state = false;
+    // adjoint workspace
+    int nck = 0, seg = 0;
+    double4* ckpt = nullptr;
+    double4* segpos = nullptr;
+    double4* wpos = nullptr;
+    double4* wvel = nullptr;
+    float4* wsnap[2] = {nullptr, nullptr};
+    float4* jrec = nullptr;
+    double *xbar = nullptr, *vbar = nullptr, *mbar = nullptr, *tgt = nullptr, *loss = nullptr;
+    // captured graph
+    hipGraphExec_t exec = nullptr;
+    int g_nsteps = -1, g_ckpt = -1, g_w = -1;
+    bool g_adj = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    BatchGeom geom(int wi) const
+    {
+        return BatchGeom{d_off, d_n, d_chunks, d_items[wi], d_start[wi]};
+    }
+};
+
+namespace {
+
+void release_adj(utcnb_batch* b)
+{
+    dfree(b->ckpt);
+    dfree(b->segpos);
+    dfree(b->wpos);
+    dfree(b->wvel);
+    dfree(b->wsnap[0]);
+    dfree(b->wsnap[1]);
+    dfree(b->jrec);
+    dfree(b->xbar);
+    dfree(b->vbar);
+    dfree(b->mbar);
+    dfree(b->tgt);
+    dfree(b->loss);
+    b->nck = b->seg = 0;
+}
+
+void release(utcnb_batch* b)
+{
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    release_adj(b);
+    dfree(b->pos);
+    dfree(b->vel);
+    dfree(b->snap[0]);
+    dfree(b->snap[1]);
+    dfree(b->d_off);
+    dfree(b->d_n);
+    dfree(b->d_chunks);
+    for (int w = 0; w < 3; ++w) {
+        dfree(b->d_items[w]);
+        dfree(b->d_start[w]);
+    }
+    if (b->ev0)
+        (void)hipEventDestroy(b->ev0);
+    if (b->ev1)
+        (void)hipEventDestroy(b->ev1);
+    if (b->own_stream && b->stream)
+        (void)hipStreamDestroy(b->stream);
+}
+
+// Longest-processing-time schedule of (system, tile) items over `blocks`
+// workgroups for W waves per item.  Item cost: ceil(chunks / W) source chunks
+// per wave plus one for the reduction and epilogue.
+void schedule(const utcnb_batch* b, int W, int blocks, std::vector<int2>& items,
+              std::vector<int>& start)
+{
+    struct It {
+        long long cost;
+        int s, t;
+    };
+    std::vector<It> all;
+    for (int s = 0; s < b->nsys; ++s) {
+        const int C = b->npad[s] / kChunk;
+        const long long cost = (C + W - 1) / W + 1;
+        for (int t = 0; t < (b->n[s] + 63) / 64; ++t)
+            all.push_back({cost, s, t});
+    }
+    std::stable_sort(all.begin(), all.end(), [](const It& a, const It& c) { return a.cost > c.cost; });
+    using Load = std::pair<long long, int>;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (int k = 0; k < blocks; ++k)
+        heap.push({0, k});
+    std::vector<std::vector<int2>> per(blocks);
+    for (const It& it : all) {
+        Load l = heap.top();
+        heap.pop();
+        per[l.second].push_back(make_int2(it.s, it.t));
+        heap.push({l.first + it.cost, l.second});
+    }
+    items.clear();
+    start.assign(1, 0);
+    for (int k = 0; k < blocks; ++k) {
+        items.insert(items.end(), per[k].begin(), per[k].end());
+        start.push_back((int)items.size());
+    }
+}
+
+int adj_workspace(utcnb_batch* b, int nck, int seg)
+{
+    if (b->xbar && b->nck >= nck && b->seg >= seg)
+        return UTCNB_OK;
+    release_adj(b);
+    const size_t T = (size_t)b->total;
+    int rc = dalloc(&b->ckpt, 2 * T * (size_t)nck) || dalloc(&b->segpos, T * (size_t)seg) ||
+             dalloc(&b->wpos, T) || dalloc(&b->wvel, T) || dalloc(&b->wsnap[0], T) ||
+             dalloc(&b->wsnap[1], T) || dalloc(&b->jrec, 2 * T) || dalloc(&b->xbar, 3 * T) ||
+             dalloc(&b->vbar, 3 * T) || dalloc(&b->mbar, T) || dalloc(&b->tgt, 3 * T) ||
+             dalloc(&b->loss, (size_t)b->nsys);
+    if (rc) {
+        release_adj(b);
+        return fail(UTCNB_ENOMEM, "utcnb_batch_run: adjoint workspace allocation failed");
+    }
+    b->nck = nck;
+    b->seg = seg;
+    // the step kernels write real bodies only: pad the work state once
+    hipError_t e = launch_batch_pad(b->geom(0), b->nsys, b->wpos, b->wvel, b->wsnap[0], b->wsnap[1],
+                                    b->stream);
+    if (e != hipSuccess)
+        return fail(UTCNB_EHIP, std::string("utcnb_batch_run: padding: ") + hipGetErrorString(e));
+    return UTCNB_OK;
+}
+
+// Tile id (single-system table) of K = 1 with the largest source split <= 16
+// that keeps whole 64-body chunks per wave.
+int per_system_tile(int npad)
+{
+    const int C = npad / kChunk;
+    static const int ids[5] = {0, 3, 5, 7, 9};  // k1s1 k1s2 k1s4 k1s8 k1s16
+    int best = 0;
+    for (int e = 1; e < 5; ++e)
+        if (C % (1 << e) == 0)
+            best = e;
+    return ids[best];
+}
+
+// Enqueue one run (forward nsteps, optionally the adjoint) on b->stream.
+// mode 0: persistent batched kernels with W = kWs[wi]; mode 1: per-system launches.
+hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode, int wi)
+{
+    hipStream_t st = b->stream;
+    const float eps2 = (float)b->cfg.eps2;
+    const double dt = b->cfg.dt, damp = b->cfg.damping;
+    const int kick = b->cfg.kick;
+    const size_t T = (size_t)b->total;
+    const size_t sb = T * sizeof(double4);
+    hipError_t e;
+#define ENQ(x)                     \
+    do {                           \
+        if ((e = (x)) != hipSuccess) \
+            return e;              \
+    } while (0)
+
+    auto step = [&](const float4* so, float4* sn, double4* p, double4* v) -> hipError_t {
+        if (mode == 0)
+            return launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], so, sn, p, v, eps2, dt,
+                                     damp, kick, st);
+        for (int s = 0; s < b->nsys; ++s) {
+            StepArgs a{};
+            a.n = b->n[s];
+            a.n_pad = b->npad[s];
+            a.eps2f = eps2;
+            a.eps2 = b->cfg.eps2;
+            a.dt = dt;
+            a.damping = damp;
+            a.kick = kick;
+            a.newton = false;
+            a.tile = per_system_tile(b->npad[s]);
+            const size_t o = (size_t)b->off[s];
+            hipError_t r = launch_mixed_step(a, so + o, sn + o, p + o, v + o, st);
+            if (r != hipSuccess)
+                return r;
+        }
+        return hipSuccess;
+    };
+    auto vjp = [&]() -> hipError_t {
+        if (mode == 0)
+            return launch_batch_vjp(kWs[wi], b->geom(wi), b->nblocks[wi], b->jrec, eps2, b->xbar,
+                                    b->mbar, st);
+        for (int s = 0; s < b->nsys; ++s) {
+            const size_t o = (size_t)b->off[s];
+            hipError_t r = launch_vjp_f32(b->jrec + 2 * o, b->n[s], b->npad[s], b->cfg.eps2,
+                                          b->xbar + 3 * o, b->mbar + o, b->cu_count, st);
+            if (r != hipSuccess)
+                return r;
+        }
+        return hipSuccess;
+    };
+
+    // every run starts from the FP64 state, so a replayed graph is consistent
+    ENQ(launch_snapshot(b->pos, b->snap[0], (int)T, st));
+    int c = 0;
+    const int k = ckpt < nsteps ? ckpt : nsteps;
+    for (int t = 0; t < nsteps; ++t) {
+        if (adj && t % k == 0) {
+            double4* ck = b->ckpt + 2 * T * (size_t)(t / k);
+            ENQ(hipMemcpyAsync(ck, b->pos, sb, hipMemcpyDeviceToDevice, st));
+            ENQ(hipMemcpyAsync(ck + T, b->vel, sb, hipMemcpyDeviceToDevice, st));
+        }
+        ENQ(step(b->snap[c], b->snap[1 - c], b->pos, b->vel));
+        c = 1 - c;
+    }
+    if (!adj)
+        return hipSuccess;
+    ENQ(launch_batch_loss(b->geom(0), b->nsys, b->pos, b->tgt, b->xbar, b->vbar, b->mbar, b->loss, st));
+    const int nck = (nsteps + k - 1) / k;
+    for (int ci = nck - 1; ci >= 0; --ci) {
+        const int s0 = ci * k, s1 = (s0 + k < nsteps) ? s0 + k : nsteps;
+        const double4* ck = b->ckpt + 2 * T * (size_t)ci;
+        ENQ(hipMemcpyAsync(b->wpos, ck, sb, hipMemcpyDeviceToDevice, st));
+        ENQ(hipMemcpyAsync(b->wvel, ck + T, sb, hipMemcpyDeviceToDevice, st));
+        ENQ(launch_snapshot(b->wpos, b->wsnap[0], (int)T, st));
+        int wc = 0;
+        for (int t = s0; t < s1; ++t) {
+            ENQ(hipMemcpyAsync(b->segpos + T * (size_t)(t - s0), b->wpos, sb,
+                               hipMemcpyDeviceToDevice, st));
+            if (t + 1 < s1) {
+                ENQ(step(b->wsnap[wc], b->wsnap[1 - wc], b->wpos, b->wvel));
+                wc = 1 - wc;
+            }
+        }
+        for (int t = s1 - 1; t >= s0; --t) {
+            ENQ(launch_adjoint_pre(true, b->segpos + T * (size_t)(t - s0), nullptr, (int)T, dt, damp,
+                                   kick, b->xbar, b->vbar, b->mbar, b->jrec, st));
+            ENQ(vjp());
+        }
+    }
+#undef ENQ
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int utcnb_batch_create(int nsys, const int32_t* sizes, const utcnb_config* cfg, void* stream,
+                       utcnb_batch** out)
+{
+    if (!out || !sizes || !cfg || nsys < 1)
+        return fail(UTCNB_EINVAL, "utcnb_batch_create: bad argument");
+    *out = nullptr;
+    if (cfg->precision != UTCNB_PREC_MIXED)
+        return fail(UTCNB_EINVAL, "utcnb_batch_create: batched systems run in MIXED precision");
+    if (cfg->kick != UTCNB_KICK_PHYSICAL && cfg->kick != UTCNB_KICK_REFERENCE)
+        return fail(UTCNB_EINVAL, "utcnb_batch_create: unknown kick mode");
+    if (!(cfg->eps2 > 0.0))
+        return fail(UTCNB_EINVAL, "utcnb_batch_create: eps2 must be > 0");
+    if (cfg->tile < 0 || cfg->tile > 3)
+        return fail(UTCNB_EINVAL, "utcnb_batch_create: tile must be 0 (tune) or 1..3");
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return fail(UTCNB_ENODEV, "utcnb_batch_create: no HIP device");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(UTCNB_ENODEV, "utcnb_batch_create: built for gfx950 only");
+    utcnb_batch* b = new (std::nothrow) utcnb_batch();
+    if (!b)
+        return fail(UTCNB_ENOMEM, "utcnb_batch_create: out of host memory");
+    b->cfg = *cfg;
+    b->cu_count = prop.multiProcessorCount;
+    b->nsys = nsys;
+    long long o = 0;
+    for (int s = 0; s < nsys; ++s) {
+        if (sizes[s] < 1 || sizes[s] > (1 << 24)) {
+            delete b;
+            return fail(UTCNB_EINVAL, "utcnb_batch_create: system sizes must be in [1, 2^24]");
+        }
+        b->n.push_back(sizes[s]);
+        b->npad.push_back((sizes[s] + kChunk - 1) / kChunk * kChunk);
+        b->off.push_back((int)o);
+        o += b->npad.back();
+        b->total_real += sizes[s];
+    }
+    if (o > (1LL << 30)) {
+        delete b;
+        return fail(UTCNB_EINVAL, "utcnb_batch_create: too many bodies in total");
+    }
+    b->total = o;
+    int rc = UTCNB_OK;
+    if (stream) {
+        b->stream = (hipStream_t)stream;
+    } else if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess) {
+        b->own_stream = true;  // capture needs a stream other than the null stream
+    } else {
+        rc = fail(UTCNB_EHIP, "utcnb_batch_create: hipStreamCreate failed");
+    }
+    const size_t T = (size_t)b->total;
+    if (!rc)
+        rc = dalloc(&b->pos, T) || dalloc(&b->vel, T) || dalloc(&b->snap[0], T) ||
+             dalloc(&b->snap[1], T) || dalloc(&b->d_off, (size_t)nsys) ||
+             dalloc(&b->d_n, (size_t)nsys) || dalloc(&b->d_chunks, (size_t)nsys);
+    if (!rc) {
+        std::vector<int> chunks(nsys);
+        for (int s = 0; s < nsys; ++s)
+            chunks[s] = b->npad[s] / kChunk;
+        hipError_t e = hipMemcpy(b->d_off, b->off.data(), sizeof(int) * nsys, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_n, b->n.data(), sizeof(int) * nsys, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_chunks, chunks.data(), sizeof(int) * nsys, hipMemcpyHostToDevice);
+        for (int wi = 0; wi < 3 && e == hipSuccess; ++wi) {
+            long long nitems = 0;
+            for (int s = 0; s < nsys; ++s)
+                nitems += (b->n[s] + 63) / 64;
+            const int W = kWs[wi];
+            int blocks = (int)std::min<long long>(nitems, (long long)b->cu_count * 32 / W);
+            std::vector<int2> items;
+            std::vector<int> start;
+            schedule(b, W, blocks, items, start);
+            b->nblocks[wi] = blocks;
+            if (dalloc(&b->d_items[wi], items.size()) || dalloc(&b->d_start[wi], start.size())) {
+                e = hipErrorOutOfMemory;
+                break;
+            }
+            e = hipMemcpy(b->d_items[wi], items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(b->d_start[wi], start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice);
+        }
+        if (e == hipSuccess)
+            e = launch_batch_pad(b->geom(0), nsys, b->pos, b->vel, b->snap[0], b->snap[1], b->stream);
+        if (e == hipSuccess && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess))
+            e = hipErrorUnknown;
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess)
+            rc = fail(UTCNB_EHIP, std::string("utcnb_batch_create: ") + hipGetErrorString(e));
+    }
+    if (cfg->tile > 0)
+        b->wsel = cfg->tile - 1;
+    if (rc) {
+        release(b);
+        delete b;
+        return rc < 0 ? rc : UTCNB_ENOMEM;
+    }
+    *out = b;
+    return UTCNB_OK;
+}
+
+int utcnb_batch_destroy(utcnb_batch* b)
+{
+    if (!b)
+        return UTCNB_OK;
+    (void)hipStreamSynchronize(b->stream);
+    release(b);
+    delete b;
+    return UTCNB_OK;
+}
+
+int utcnb_batch_set_state(utcnb_batch* b, const double* pos4, const double* vel4)
+{
+    if (!b || !pos4 || !vel4)
+        return fail(UTCNB_EINVAL, "utcnb_batch_set_state: null argument");
+    const size_t T = (size_t)b->total;
+    std::vector<double> hp(4 * T), hv(4 * T);
+    size_t src = 0;
+    for (int s = 0; s < b->nsys; ++s) {
+        const size_t o = (size_t)b->off[s];
+        for (int t = 0; t < b->npad[s]; ++t) {
+            double* p = &hp[4 * (o + t)];
+            double* v = &hv[4 * (o + t)];
+            if (t < b->n[s]) {
+                std::memcpy(p, pos4 + 4 * src, 4 * sizeof(double));
+                std::memcpy(v, vel4 + 4 * src, 4 * sizeof(double));
+                ++src;
+            } else {
+                p[0] = p[1] = p[2] = kPadCoord;
+                p[3] = 0.0;
+                v[0] = v[1] = v[2] = 0.0;
+                v[3] = 1.0;
+            }
+        }
+    }
+    UTCNB_HIP(hipMemcpyAsync(b->pos, hp.data(), 4 * T * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    UTCNB_HIP(hipMemcpyAsync(b->vel, hv.data(), 4 * T * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    UTCNB_HIP(hipStreamSynchronize(b->stream));
+    b->has_state = true;
+    return UTCNB_OK;
+}
+
+int utcnb_batch_get_state(utcnb_batch* b, double* pos4, double* vel4)
+{
+    if (!b)
+        return fail(UTCNB_EINVAL, "utcnb_batch_get_state: null handle");
+    if (!b->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_batch_get_state: no state set");
+    const size_t T = (size_t)b->total;
+    std::vector<double> hp(4 * T), hv(4 * T);
+    UTCNB_HIP(hipMemcpyAsync(hp.data(), b->pos, 4 * T * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    UTCNB_HIP(hipMemcpyAsync(hv.data(), b->vel, 4 * T * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    UTCNB_HIP(hipStreamSynchronize(b->stream));
+    size_t dst = 0;
+    for (int s = 0; s < b->nsys; ++s)
+        for (int t = 0; t < b->n[s]; ++t, ++dst) {
+            const size_t i = (size_t)b->off[s] + t;
+            if (pos4)
+                std::memcpy(pos4 + 4 * dst, &hp[4 * i], 4 * sizeof(double));
+            if (vel4)
+                std::memcpy(vel4 + 4 * dst, &hv[4 * i], 4 * sizeof(double));
+        }
+    return UTCNB_OK;
+}
+
+// Pick W by timing one forward step of every tile on a copy of the state.
+static int tune_w(utcnb_batch* b)
+{
+    const size_t T = (size_t)b->total;
+    double4 *sp = nullptr, *sv = nullptr;
+    if (dalloc(&sp, T) || dalloc(&sv, T)) {
+        dfree(sp);
+        return fail(UTCNB_ENOMEM, "utcnb_batch_run: tuning buffers");
+    }
+    UTCNB_HIP(hipMemcpyAsync(sp, b->pos, T * sizeof(double4), hipMemcpyDeviceToDevice, b->stream));
+    UTCNB_HIP(hipMemcpyAsync(sv, b->vel, T * sizeof(double4), hipMemcpyDeviceToDevice, b->stream));
+    float best = 1e30f;
+    int bw = 1;
+    for (int wi = 0; wi < 3; ++wi) {
+        UTCNB_HIP(launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], b->snap[0], b->snap[1], b->pos,
+                                    b->vel, (float)b->cfg.eps2, b->cfg.dt, b->cfg.damping, b->cfg.kick,
+                                    b->stream));
+        UTCNB_HIP(hipEventRecord(b->ev0, b->stream));
+        for (int r = 0; r < 3; ++r)
+            UTCNB_HIP(launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], b->snap[0], b->snap[1],
+                                        b->pos, b->vel, (float)b->cfg.eps2, b->cfg.dt, b->cfg.damping,
+                                        b->cfg.kick, b->stream));
+        UTCNB_HIP(hipEventRecord(b->ev1, b->stream));
+        UTCNB_HIP(hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        UTCNB_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        if (ms < best) {
+            best = ms;
+            bw = wi;
+        }
+    }
+    UTCNB_HIP(hipMemcpyAsync(b->pos, sp, T * sizeof(double4), hipMemcpyDeviceToDevice, b->stream));
+    UTCNB_HIP(hipMemcpyAsync(b->vel, sv, T * sizeof(double4), hipMemcpyDeviceToDevice, b->stream));
+    UTCNB_HIP(hipStreamSynchronize(b->stream));
+    dfree(sp);
+    dfree(sv);
+    b->wsel = bw;
+    return UTCNB_OK;
+}
+
+int utcnb_batch_run(utcnb_batch* b, int nsteps, int mode, int ckpt_every, const double* target3,
+                    double* losses, double* dx0, double* dv0, double* dm, double* ms)
+{
+    if (!b || nsteps < 1 || mode < 0 || mode > 2)
+        return fail(UTCNB_EINVAL, "utcnb_batch_run: bad argument");
+    if (!b->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_batch_run: no state set");
+    const bool adj = target3 != nullptr;
+    if (adj) {
+        if (!losses || !dx0 || !dv0 || !dm || ckpt_every < 1)
+            return fail(UTCNB_EINVAL, "utcnb_batch_run: adjoint outputs and ckpt_every >= 1 required");
+        if (b->cfg.kick != UTCNB_KICK_PHYSICAL)
+            return fail(UTCNB_EINVAL, "utcnb_batch_run: the batched adjoint supports the physical kick");
+        const int k = ckpt_every < nsteps ? ckpt_every : nsteps;
+        int rc = adj_workspace(b, (nsteps + k - 1) / k, k);
+        if (rc)
+            return rc;
+        // targets, compact host layout -> padded device layout
+        const size_t T = (size_t)b->total;
+        std::vector<double> ht(3 * T, 0.0);
+        size_t src = 0;
+        for (int s = 0; s < b->nsys; ++s)
+            for (int t = 0; t < b->n[s]; ++t, ++src)
+                std::memcpy(&ht[3 * ((size_t)b->off[s] + t)], target3 + 3 * src, 3 * sizeof(double));
+        UTCNB_HIP(hipMemcpyAsync(b->tgt, ht.data(), 3 * T * sizeof(double), hipMemcpyHostToDevice, b->stream));
+        UTCNB_HIP(hipStreamSynchronize(b->stream));
+    }
+    if (mode != 1 && b->wsel < 0) {
+        int rc = tune_w(b);
+        if (rc)
+            return rc;
+    }
+    const int wi = b->wsel < 0 ? 1 : b->wsel;
+    if (mode == 0) {
+        if (!b->exec || b->g_nsteps != nsteps || b->g_ckpt != ckpt_every || b->g_adj != adj ||
+            b->g_w != wi) {
+            if (b->exec)
+                (void)hipGraphExecDestroy(b->exec);
+            b->exec = nullptr;
+            hipGraph_t graph = nullptr;
+            UTCNB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = enqueue_run(b, nsteps, adj ? ckpt_every : nsteps, adj, 0, wi);
+            hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (graph)
+                    (void)hipGraphDestroy(graph);
+                return fail(UTCNB_EHIP, std::string("utcnb_batch_run: capture: ") +
+                                            hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess)
+                return fail(UTCNB_EHIP, std::string("utcnb_batch_run: instantiate: ") + hipGetErrorString(e));
+            b->g_nsteps = nsteps;
+            b->g_ckpt = ckpt_every;
+            b->g_adj = adj;
+            b->g_w = wi;
+        }
+        UTCNB_HIP(hipEventRecord(b->ev0, b->stream));
+        UTCNB_HIP(hipGraphLaunch(b->exec, b->stream));
+        UTCNB_HIP(hipEventRecord(b->ev1, b->stream));
+    } else {
+        // mode 1: per-system launches; mode 2: the persistent kernels, eagerly
+        UTCNB_HIP(hipEventRecord(b->ev0, b->stream));
+        hipError_t e = enqueue_run(b, nsteps, adj ? ckpt_every : nsteps, adj, mode == 2 ? 0 : 1, wi);
+        if (e != hipSuccess)
+            return fail(UTCNB_EHIP, std::string("utcnb_batch_run: ") + hipGetErrorString(e));
+        UTCNB_HIP(hipEventRecord(b->ev1, b->stream));
+    }
+    UTCNB_HIP(hipEventSynchronize(b->ev1));
+    if (ms) {
+        float f = 0.f;
+        UTCNB_HIP(hipEventElapsedTime(&f, b->ev0, b->ev1));
+        *ms = f;
+    }
+    if (adj) {
+        const size_t T = (size_t)b->total;
+        std::vector<double> hx(3 * T), hv(3 * T), hm(T);
+        UTCNB_HIP(hipMemcpyAsync(hx.data(), b->xbar, 3 * T * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        UTCNB_HIP(hipMemcpyAsync(hv.data(), b->vbar, 3 * T * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        UTCNB_HIP(hipMemcpyAsync(hm.data(), b->mbar, T * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        UTCNB_HIP(hipMemcpyAsync(losses, b->loss, b->nsys * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        UTCNB_HIP(hipStreamSynchronize(b->stream));
+        size_t dst = 0;
+        for (int s = 0; s < b->nsys; ++s)
+            for (int t = 0; t < b->n[s]; ++t, ++dst) {
+                const size_t i = (size_t)b->off[s] + t;
+                std::memcpy(dx0 + 3 * dst, &hx[3 * i], 3 * sizeof(double));
+                std::memcpy(dv0 + 3 * dst, &hv[3 * i], 3 * sizeof(double));
+                dm[dst] = hm[i];
+            }
+    }
+    return UTCNB_OK;
+}
+
+}  // extern "C"

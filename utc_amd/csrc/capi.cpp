@@ -53,6 +53,22 @@ struct utcnb_system {
     void* stage = nullptr;       // conversion staging, 4n * 8 bytes
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int last_launches = 0;
+    // adjoint workspace (allocated on first use, grown on demand)
+    struct Adj {
+        int nck = 0, seg = 0;
+        double4* ckpt = nullptr;     // nck * 2n (pos, vel)
+        double4* segpos = nullptr;   // seg * n
+        double4* wpos[2] = {nullptr, nullptr};  // n_pad (FP64 work positions)
+        double4* wvel = nullptr;     // n
+        float4* wsnap[2] = {nullptr, nullptr};  // n_pad (MIXED work snapshots)
+        void* jrec = nullptr;        // 2 * n_pad records (float4 or double4)
+        void* fsrc = nullptr;        // n_pad padded field source (float4 or double4)
+        double* xbar = nullptr;      // 3n
+        double* vbar = nullptr;      // 3n
+        double* mbar = nullptr;      // n
+        double* tgt = nullptr;       // 3n
+        double* part = nullptr;      // loss partials + loss
+    } adj;
     bool timed = false;
     int cu_count = 256;
 };
@@ -98,8 +114,32 @@ void dfree(T*& p)
     p = nullptr;
 }
 
+void release_adjoint(utcnb_system* s)
+{
+    auto& a = s->adj;
+    dfree(a.ckpt);
+    dfree(a.segpos);
+    dfree(a.wpos[0]);
+    dfree(a.wpos[1]);
+    dfree(a.wvel);
+    dfree(a.wsnap[0]);
+    dfree(a.wsnap[1]);
+    if (a.jrec)
+        (void)hipFree(a.jrec);
+    if (a.fsrc)
+        (void)hipFree(a.fsrc);
+    a.jrec = a.fsrc = nullptr;
+    dfree(a.xbar);
+    dfree(a.vbar);
+    dfree(a.mbar);
+    dfree(a.tgt);
+    dfree(a.part);
+    a.nck = a.seg = 0;
+}
+
 void release(utcnb_system* s)
 {
+    release_adjoint(s);
     dfree(s->p32[0]);
     dfree(s->p32[1]);
     dfree(s->v32);
@@ -537,6 +577,167 @@ int utcnb_accel(utcnb_system* s, double* acc3)
     UTCNB_HIP(hipMemcpyAsync(acc3, s->scratch, sizeof(double) * 3 * (size_t)s->n,
                              hipMemcpyDeviceToHost, s->stream));
     UTCNB_HIP(hipStreamSynchronize(s->stream));
+    return UTCNB_OK;
+}
+
+
+// ---- adjoint -------------------------------------------------------------------
+static int adjoint_workspace(utcnb_system* s, int nck, int seg)
+{
+    auto& a = s->adj;
+    const size_t n = (size_t)s->n, np = (size_t)s->n_pad;
+    const bool f32 = s->cfg.precision == UTCNB_PREC_MIXED;
+    if (a.xbar && a.nck >= nck && a.seg >= seg)
+        return UTCNB_OK;
+    release_adjoint(s);
+    int rc = dalloc(&a.ckpt, 2 * n * (size_t)nck) || dalloc(&a.segpos, n * (size_t)seg) ||
+             dalloc(&a.wpos[0], np) || dalloc(&a.wpos[1], np) || dalloc(&a.wvel, n) ||
+             dalloc(&a.xbar, 3 * n) || dalloc(&a.vbar, 3 * n) || dalloc(&a.mbar, n) ||
+             dalloc(&a.tgt, 3 * n) || dalloc(&a.part, (n + 255) / 256 + 1);
+    if (!rc && f32)
+        rc = dalloc(&a.wsnap[0], np) || dalloc(&a.wsnap[1], np);
+    if (!rc) {
+        const size_t rb = f32 ? sizeof(float4) : sizeof(double4);
+        if (hipMalloc(&a.jrec, 2 * np * rb) != hipSuccess || hipMalloc(&a.fsrc, np * rb) != hipSuccess)
+            rc = 1;
+    }
+    if (rc) {
+        release_adjoint(s);
+        return fail(UTCNB_ENOMEM, "utcnb_adjoint: workspace allocation failed");
+    }
+    a.nck = nck;
+    a.seg = seg;
+    hipError_t e = launch_jrec_pad(f32, a.jrec, s->n, s->n_pad, s->stream);
+    if (e == hipSuccess && f32)
+        e = launch_pad_f32((float4*)a.fsrc, s->n, s->n_pad, s->stream);
+    if (e == hipSuccess && f32)
+        e = launch_pad_f32(a.wsnap[0], s->n, s->n_pad, s->stream);
+    if (e == hipSuccess && f32)
+        e = launch_pad_f32(a.wsnap[1], s->n, s->n_pad, s->stream);
+    if (e == hipSuccess && !f32)
+        e = launch_pad_f64((double4*)a.fsrc, s->n, s->n_pad, s->stream);
+    if (e == hipSuccess)
+        e = launch_pad_f64(a.wpos[0], s->n, s->n_pad, s->stream);
+    if (e == hipSuccess)
+        e = launch_pad_f64(a.wpos[1], s->n, s->n_pad, s->stream);
+    if (e != hipSuccess)
+        return fail(UTCNB_EHIP, std::string("utcnb_adjoint: padding: ") + hipGetErrorString(e));
+    return UTCNB_OK;
+}
+
+// One forward step of the adjoint's work state (wpos/wvel, with wsnap for MIXED).
+static hipError_t work_step(utcnb_system* s, int& wc)
+{
+    auto& a = s->adj;
+    hipError_t e;
+    if (s->cfg.precision == UTCNB_PREC_MIXED) {
+        e = launch_mixed_step(s->args, a.wsnap[wc], a.wsnap[1 - wc], a.wpos[0], a.wvel, s->stream);
+    } else {
+        e = launch_fp64_step(s->args, a.wpos[wc], a.wpos[1 - wc], a.wvel, s->stream);
+    }
+    wc = 1 - wc;
+    return e;
+}
+
+int utcnb_adjoint(utcnb_system* s, int nsteps, int ckpt_every, const double* target3,
+                             double* loss, double* dx0, double* dv0, double* dm)
+{
+    if (!s || !target3 || !loss || !dx0 || !dv0 || !dm)
+        return fail(UTCNB_EINVAL, "utcnb_adjoint: null argument");
+    if (s->cfg.precision != UTCNB_PREC_MIXED && s->cfg.precision != UTCNB_PREC_FP64)
+        return fail(UTCNB_EINVAL, "utcnb_adjoint: precision must be MIXED or FP64");
+    if (nsteps < 1 || ckpt_every < 1)
+        return fail(UTCNB_EINVAL, "utcnb_adjoint: nsteps and ckpt_every must be >= 1");
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_adjoint: no state set");
+    const int k = ckpt_every < nsteps ? ckpt_every : nsteps;
+    const int nck = (nsteps + k - 1) / k;
+    int rc = adjoint_workspace(s, nck, k);
+    if (rc)
+        return rc;
+    auto& a = s->adj;
+    const bool mixed = s->cfg.precision == UTCNB_PREC_MIXED;
+    const size_t n = (size_t)s->n;
+    const size_t sb = n * sizeof(double4);
+    const bool refkick = s->cfg.kick == UTCNB_KICK_REFERENCE;
+    hipStream_t st = s->stream;
+    int launches = 0;
+#define ADJ_CHECK(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(UTCNB_EHIP, std::string("utcnb_adjoint: ") + #expr + ": " +     \
+                                        hipGetErrorString(e_));                          \
+    } while (0)
+
+    ADJ_CHECK(hipMemcpyAsync(a.tgt, target3, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+    ADJ_CHECK(hipEventRecord(s->ev0, st));
+    // forward sweep with checkpoints of (x, v) at steps 0, k, 2k, ...
+    for (int t = 0; t < nsteps; ++t) {
+        if (t % k == 0) {
+            double4* ck = a.ckpt + 2 * n * (size_t)(t / k);
+            ADJ_CHECK(hipMemcpyAsync(ck, cur_p64(s), sb, hipMemcpyDeviceToDevice, st));
+            ADJ_CHECK(hipMemcpyAsync(ck + n, s->v64, sb, hipMemcpyDeviceToDevice, st));
+        }
+        ADJ_CHECK(step_once(s));
+        ++launches;
+    }
+    // loss and its seed at the final state
+    double* dloss = a.part + (n + 255) / 256;
+    ADJ_CHECK(launch_loss_seed(cur_p64(s), a.tgt, s->n, a.xbar, a.vbar, a.mbar, a.part, dloss, st));
+    // reverse sweep, one checkpoint segment at a time
+    for (int c = nck - 1; c >= 0; --c) {
+        const int s0 = c * k;
+        const int s1 = (s0 + k < nsteps) ? s0 + k : nsteps;
+        const double4* ck = a.ckpt + 2 * n * (size_t)c;
+        int wc = 0;
+        ADJ_CHECK(hipMemcpyAsync(a.wpos[0], ck, sb, hipMemcpyDeviceToDevice, st));
+        ADJ_CHECK(hipMemcpyAsync(a.wvel, ck + n, sb, hipMemcpyDeviceToDevice, st));
+        if (mixed)
+            ADJ_CHECK(launch_snapshot(a.wpos[0], a.wsnap[0], s->n, st));
+        for (int t = s0; t < s1; ++t) {
+            const double4* cur = mixed ? a.wpos[0] : a.wpos[wc];
+            ADJ_CHECK(hipMemcpyAsync(a.segpos + n * (size_t)(t - s0), cur, sb,
+                                     hipMemcpyDeviceToDevice, st));
+            if (t + 1 < s1) {
+                ADJ_CHECK(work_step(s, wc));
+                ++launches;
+            }
+        }
+        for (int t = s1 - 1; t >= s0; --t) {
+            const double4* xs = a.segpos + n * (size_t)(t - s0);
+            const double* acc = nullptr;
+            if (refkick) {
+                if (mixed) {
+                    ADJ_CHECK(launch_snapshot(xs, (float4*)a.fsrc, s->n, st));
+                    ADJ_CHECK(launch_fp32_accel(s->args, (const float4*)a.fsrc, true, s->scratch, st));
+                } else {
+                    ADJ_CHECK(hipMemcpyAsync(a.fsrc, xs, sb, hipMemcpyDeviceToDevice, st));
+                    ADJ_CHECK(launch_fp64_accel(s->args, (const double4*)a.fsrc, s->scratch, st));
+                }
+                acc = s->scratch;
+                ++launches;
+            }
+            ADJ_CHECK(launch_adjoint_pre(mixed, xs, acc, s->n, s->cfg.dt, s->cfg.damping,
+                                         s->cfg.kick, a.xbar, a.vbar, a.mbar, a.jrec, st));
+            if (mixed)
+                ADJ_CHECK(launch_vjp_f32((const float4*)a.jrec, s->n, s->n_pad, s->cfg.eps2,
+                                         a.xbar, a.mbar, s->cu_count, st));
+            else
+                ADJ_CHECK(launch_vjp_f64((const double4*)a.jrec, s->n, s->n_pad, s->cfg.eps2,
+                                         a.xbar, a.mbar, s->cu_count, st));
+            ++launches;
+        }
+    }
+    ADJ_CHECK(hipEventRecord(s->ev1, st));
+    s->last_launches = launches;
+    s->timed = true;
+    ADJ_CHECK(hipMemcpyAsync(dx0, a.xbar, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    ADJ_CHECK(hipMemcpyAsync(dv0, a.vbar, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    ADJ_CHECK(hipMemcpyAsync(dm, a.mbar, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    ADJ_CHECK(hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, st));
+    ADJ_CHECK(hipStreamSynchronize(st));
+#undef ADJ_CHECK
     return UTCNB_OK;
 }
 

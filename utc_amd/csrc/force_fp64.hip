@@ -119,10 +119,10 @@ static hipError_t launch64(const double4* src, int n, int n_pad, double eps2, co
                            hipStream_t st)
 {
     const int waves = (n + 63) / 64;
-    if (waves >= 2048)
-        return launch64_tile<1>(src, n, n_pad, eps2, epi, st);
-    if (waves * 4 >= 2048)
+    if (waves >= 4096)
         return launch64_tile<4>(src, n, n_pad, eps2, epi, st);
+    if (waves * 8 >= 4096 * 4)
+        return launch64_tile<8>(src, n, n_pad, eps2, epi, st);
     return launch64_tile<16>(src, n, n_pad, eps2, epi, st);
 }
 

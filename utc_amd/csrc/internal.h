@@ -83,6 +83,39 @@ hipError_t launch_ref_accel_f32(const float4* src, int n, float eps2, double* ac
 hipError_t launch_ref_accel_f64(const double4* src, int n, double eps2, double* acc3,
                                 hipStream_t st);
 
+// ---- adjoint (adjoint.hip) ------------------------------------------------
+// jrec: 2 records per body {x,y,z,m}{ax,ay,az,0}, n_pad bodies.
+hipError_t launch_vjp_f32(const float4* jrec, int n, int n_pad, double eps2, double* xbar3,
+                          double* mbar, int cu_count, hipStream_t st);
+hipError_t launch_vjp_f64(const double4* jrec, int n, int n_pad, double eps2, double* xbar3,
+                          double* mbar, int cu_count, hipStream_t st);
+hipError_t launch_loss_seed(const double4* pos, const double* tgt3, int n, double* xbar3,
+                            double* vbar3, double* mbar, double* part, double* loss,
+                            hipStream_t st);
+hipError_t launch_adjoint_pre(bool fp32_records, const double4* pos, const double* acc3, int n,
+                              double dt, double damping, int kick, double* xbar3, double* vbar3,
+                              double* mbar, void* jrec, hipStream_t st);
+hipError_t launch_jrec_pad(bool fp32_records, void* jrec, int n, int n_pad, hipStream_t st);
+
+// ---- batched systems (batch.hip) ---------------------------------------------
+struct BatchGeom {
+    const int* off;    // first body of each system (device)
+    const int* n;      // bodies per system
+    const int* chunks; // padded bodies / 64
+    const int2* items; // (system, target tile) in workgroup order
+    const int* start;  // items of workgroup b: [start[b], start[b+1])
+};
+hipError_t launch_batch_step(int W, const BatchGeom& g, int nblocks, const float4* snap_old,
+                             float4* snap_new, double4* pos, double4* vel, float eps2, double dt,
+                             double damping, int kick, hipStream_t st);
+hipError_t launch_batch_vjp(int W, const BatchGeom& g, int nblocks, const float4* jrec, float eps2,
+                            double* xbar3, double* mbar, hipStream_t st);
+hipError_t launch_batch_loss(const BatchGeom& g, int nsys, const double4* pos, const double* tgt3,
+                             double* xbar3, double* vbar3, double* mbar, double* loss,
+                             hipStream_t st);
+hipError_t launch_batch_pad(const BatchGeom& g, int nsys, double4* pos, double4* vel, float4* s0,
+                            float4* s1, hipStream_t st);
+
 // ---- helpers (aux.hip) ----------------------------------------------------
 // Fill padding bodies [n, n_pad) of a float4 / double4 source array.
 hipError_t launch_pad_f32(float4* p, int n, int n_pad, hipStream_t st);
