@@ -68,7 +68,9 @@ def test_nbody_main_double_frames(tmp_path):
     lines = (tmp_path / "d.txt").read_text().splitlines()
     n = 2048  # rounded up to a multiple of the block size
     assert len(lines) == 5 * (n + 1)
-    assert lines[0] == "0.000000" and lines[n + 1] == "0.080000"
+    # the reference labels frame k with k * timestep, not with the step count
+    # (benchapps/nbody/utc/nbody_main.cc:236-238)
+    assert lines[0] == "0.000000" and lines[n + 1] == "0.016000" and lines[4 * (n + 1)] == "0.064000"
 
 
 def test_nbody_mtask_runs_concurrent_tasks(tmp_path):
