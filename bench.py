@@ -96,6 +96,39 @@ def cpu_baseline(c, n, threads, rows):
     }
 
 
+def batched_config5(U, nsys=256, steps=20, seed=0):
+    """256 Plummer systems, n ~ round(exp(U[ln 256, ln 8192])), each with its
+    own seed, eps = 0.01, dt = 1e-3, 20 steps; forward and forward+adjoint, one
+    captured HIP graph of persistent kernels vs per-system launches."""
+    from tests.nbcases import config5_sizes
+    sizes = config5_sizes(nsys, seed)
+    pos, vel = [], []
+    for k, n in enumerate(sizes):
+        p, v = U.generate(U.GEN_PLUMMER, n, 1000 + k)
+        pos.append(p)
+        vel.append(v)
+    pos, vel = np.concatenate(pos), np.concatenate(vel)
+    tgt = U.generate(U.GEN_TARGETS, len(pos), 77, 0.5)
+    pairs = float(sum(n * n for n in sizes)) * steps
+    out = {"systems": nsys, "bodies": int(sum(sizes)), "pairs_per_step": pairs / steps, "steps": steps}
+    with U.Batch(sizes, 0.01 ** 2, 1e-3) as b:
+        for label, mode, adj in (("graph_fwd", 0, False), ("per_system_fwd", 1, False),
+                                 ("graph_fwd_adj", 0, True), ("per_system_fwd_adj", 1, True)):
+            b.set_state(pos, vel)
+            if adj:
+                b.run(steps, mode, 10, tgt)  # warm-up: capture / workspace / tuning
+            else:
+                b.run(steps, mode)
+            b.set_state(pos, vel)
+            ms = b.run(steps, mode, 10, tgt)[0] if adj else b.run(steps, mode)
+            out[label + "_ms"] = ms
+            out[label + "_interactions_per_s"] = pairs / (ms * 1e-3)
+            log(f"batched {label}: {ms:.2f} ms, {pairs / (ms * 1e-3):.4e} interactions/s")
+    out["graph_speedup_fwd"] = out["per_system_fwd_ms"] / out["graph_fwd_ms"]
+    out["graph_speedup_fwd_adj"] = out["per_system_fwd_adj_ms"] / out["graph_fwd_adj_ms"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +138,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8192)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp32"], default=None,
                     help="time one path only and print its line (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -201,7 +235,7 @@ def main():
             tgt = U.generate(U.GEN_TARGETS, n, 0x5EED7A56E7, 0.5)
             sa = U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON)
             sa.set_state(c["pos"], c["vel"])
-            sa.adjoint(2, 1, tgt)
+            sa.adjoint(K, 10, tgt)  # warm-up with the timed shape (allocates the workspace)
             sa.set_state(c["pos"], c["vel"])
             sa.sync()
             t0 = time.perf_counter()
@@ -221,6 +255,15 @@ def main():
         except Exception as e:  # reported, never hidden
             fwd_adj = {"error": str(e)}
             log(f"fwd+adjoint failed: {e}")
+
+    # ---- batched systems (config 5) ------------------------------------------------
+    batch = None
+    if not args.no_batch:
+        try:
+            batch = batched_config5(U)
+        except Exception as e:  # reported, never hidden
+            batch = {"error": str(e)}
+            log(f"batched run failed: {e}")
 
     # ---- roofline of the force kernel ------------------------------------------
     m = results["mixed"]
@@ -272,6 +315,7 @@ def main():
         "accuracy_vs_fp64": acc,
         "energy": energy,
         "fwd_adjoint": fwd_adj,
+        "batched": batch,
         "fraction_of_fp32_peak": achieved / PEAK_FP32,
         "roofline": roofline,
         "cpu_baseline": cpu,

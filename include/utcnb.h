@@ -38,8 +38,8 @@ extern "C" {
                                   dtype: sqrt then divide, no fused multiply-add,
                                   j summed in order 0..N-1; bit-identical to
                                   benchapps/nbody/seq/bodysystem.{h:70-96,cc:103-181} */
-#define UTCNB_PREC_FP32 1      /* plain FP32: v_rsq + Newton step, FMA, FP32 sums
-                                  and FP32 state */
+#define UTCNB_PREC_FP32 1      /* plain FP32: v_rsq_f32 (optionally refined by one
+                                  Newton step), FMA, FP32 sums and FP32 state */
 #define UTCNB_PREC_MIXED 2     /* FP32 pairs with compensated per-body sums;
                                   FP64 state and integrator */
 #define UTCNB_PREC_FP64 3      /* FP64 pairs (correctly rounded sqrt/divide),
@@ -56,7 +56,8 @@ extern "C" {
 #define UTCNB_DEVICE 1
 
 /* config flags */
-#define UTCNB_FLAG_NO_NEWTON 1 /* FP32/MIXED: use v_rsq_f32 without the Newton step */
+#define UTCNB_FLAG_NO_NEWTON 1 /* FP32/MIXED: use v_rsq_f32 without the Newton step
+                                  (same measured accuracy, ~20% faster) */
 
 /* ---- kernel-level entry points (device pointers, reference layout) ------ */
 /*
@@ -89,7 +90,8 @@ typedef struct {
     int32_t kick;        /* UTCNB_KICK_* */
     int32_t state_dtype; /* UTCNB_PREC_REFERENCE only: UTCNB_F32 or UTCNB_F64 */
     int32_t flags;       /* UTCNB_FLAG_* */
-    int32_t tile;        /* force-kernel tile id, 0 = tuned default (utcnb_autotune) */
+    int32_t tile;        /* force-kernel tile id 1..utcnb_tile_count() (e.g. from
+                            utcnb_autotune); 0 = built-in default */
     double eps2;         /* softening squared, > 0 unless precision is REFERENCE */
     double damping;      /* velocity damping per step (reference default 1.0) */
     double dt;           /* time step */
