@@ -51,6 +51,7 @@ NO_CONTRACT = {"ref.hip"}
 LIB_SOURCES = [
     "ref.hip",
     "force_fp32.hip",
+    "sym.hip",
     "force_fp64.hip",
     "aux.hip",
     "adjoint.hip",

@@ -49,6 +49,8 @@ struct utcnb_system {
     double4* v64 = nullptr;
     float4* snap[2] = {nullptr, nullptr};
     double* scratch = nullptr;   // 3n doubles (field) / energy partials
+    float* ipart = nullptr;      // symmetric-tile workspace
+    float* jpart = nullptr;
     size_t scratch_bytes = 0;
     void* stage = nullptr;       // conversion staging, 4n * 8 bytes
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -149,6 +151,8 @@ void release(utcnb_system* s)
     dfree(s->snap[0]);
     dfree(s->snap[1]);
     dfree(s->scratch);
+    dfree(s->ipart);
+    dfree(s->jpart);
     if (s->stage)
         (void)hipFree(s->stage);
     s->stage = nullptr;
@@ -359,6 +363,23 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
     case UTCNB_PREC_FP64:
         rc = dalloc(&s->p64[0], np) || dalloc(&s->p64[1], np) || dalloc(&s->v64, n);
         break;
+    }
+    if (!rc && (c.precision == UTCNB_PREC_FP32 || c.precision == UTCNB_PREC_MIXED) &&
+        tile_get(s->args.tile).sym) {
+        // symmetric tiles keep one reaction partial per (wave block, wave block) pair
+        const Tile& t = tile_get(s->args.tile);
+        if (np % (size_t)(128 * t.k) != 0) {
+            rc = fail(UTCNB_EINVAL, "utcnb_create: symmetric tile needs N padded to 128*K");
+        } else {
+            const size_t jf = sym_jpart_floats((int)np, t.k);
+            if (jf * sizeof(float) > ((size_t)16 << 30))
+                rc = fail(UTCNB_EINVAL, "utcnb_create: N too large for a symmetric tile (reaction "
+                                        "partials above 16 GiB); use a one-pass tile");
+            else
+                rc = dalloc(&s->ipart, sym_ipart_floats((int)np, t.k, t.s)) || dalloc(&s->jpart, jf);
+        }
+        s->args.ipart = s->ipart;
+        s->args.jpart = s->jpart;
     }
     if (!rc) {
         s->scratch_bytes = sizeof(double) * (3 * n + 5 + 5 * ((n + 255) / 256));

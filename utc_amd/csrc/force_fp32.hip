@@ -6,8 +6,13 @@ namespace utcnb {
 
 namespace {
 const Tile kTiles[] = {
-    {1, 1, "k1s1"}, {2, 1, "k2s1"}, {4, 1, "k4s1"}, {1, 2, "k1s2"}, {2, 2, "k2s2"},
-    {1, 4, "k1s4"}, {2, 4, "k2s4"}, {1, 8, "k1s8"}, {2, 8, "k2s8"}, {1, 16, "k1s16"},
+    {1, 1, "k1s1", false},  {2, 1, "k2s1", false},  {4, 1, "k4s1", false},
+    {1, 2, "k1s2", false},  {2, 2, "k2s2", false},  {1, 4, "k1s4", false},
+    {2, 4, "k2s4", false},  {1, 8, "k1s8", false},  {2, 8, "k2s8", false},
+    {1, 16, "k1s16", false},
+    // symmetric (sym.hip): K targets per lane, R waves per wave block
+    {4, 4, "sym4r4", true}, {4, 8, "sym4r8", true}, {8, 4, "sym8r4", true},
+    {8, 8, "sym8r8", true}, {8, 16, "sym8r16", true},
 };
 }  // namespace
 
@@ -26,10 +31,17 @@ int tile_default(int64_t n, int cu_count)
     return 9;      // k1s16
 }
 
+template <class Epi>
+hipError_t launch_sym(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
+                      float* ipart, float* jpart, const Epi& epi, hipStream_t st);
+
 hipError_t launch_fp32_step(const StepArgs& a, const float4* pos_old, float4* pos_new,
                             float4* vel, hipStream_t st)
 {
     EpiFP32 epi{pos_old, pos_new, vel, (float)a.dt, (float)a.damping, a.kick};
+    if (tile_get(a.tile).sym)
+        return launch_sym(tile_get(a.tile), a.newton, pos_old, a.n, a.n_pad, a.eps2f, a.ipart, a.jpart,
+                          epi, st);
     if (a.newton)
         return launch_force<false, true>(a.tile, pos_old, a.n, a.n_pad, a.eps2f, epi, st);
     return launch_force<false, false>(a.tile, pos_old, a.n, a.n_pad, a.eps2f, epi, st);
@@ -39,6 +51,9 @@ hipError_t launch_mixed_step(const StepArgs& a, const float4* snap_old, float4* 
                              double4* pos, double4* vel, hipStream_t st)
 {
     EpiMixed epi{pos, vel, snap_new, a.dt, a.damping, a.kick};
+    if (tile_get(a.tile).sym)
+        return launch_sym(tile_get(a.tile), a.newton, snap_old, a.n, a.n_pad, a.eps2f, a.ipart, a.jpart,
+                          epi, st);
     if (a.newton)
         return launch_force<true, true>(a.tile, snap_old, a.n, a.n_pad, a.eps2f, epi, st);
     return launch_force<true, false>(a.tile, snap_old, a.n, a.n_pad, a.eps2f, epi, st);
@@ -48,6 +63,9 @@ hipError_t launch_fp32_accel(const StepArgs& a, const float4* src, bool compensa
                              double* acc3, hipStream_t st)
 {
     EpiAccel epi{acc3};
+    if (tile_get(a.tile).sym)
+        return launch_sym(tile_get(a.tile), a.newton, src, a.n, a.n_pad, a.eps2f, a.ipart, a.jpart, epi,
+                          st);
     if (compensated) {
         if (a.newton)
             return launch_force<true, true>(a.tile, src, a.n, a.n_pad, a.eps2f, epi, st);

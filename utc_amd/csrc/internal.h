@@ -34,9 +34,10 @@ inline int64_t pad_j(int64_t n) { return (n + kPadJ - 1) / kPadJ * kPadJ; }
 // ---- force-kernel tiles ---------------------------------------------------
 // A tile is (K bodies i per lane, S waves splitting the j range per workgroup).
 struct Tile {
-    int k;
-    int s;
+    int k;       // targets per lane
+    int s;       // source split (one-pass tiles) or waves per wave block (symmetric tiles)
     const char* name;
+    bool sym;    // evaluates each unordered pair once (sym.hip)
 };
 int tile_count();
 const Tile& tile_get(int id);
@@ -53,7 +54,12 @@ struct StepArgs {
     int kick;       // UTCNB_KICK_*
     bool newton;
     int tile;       // tile id
+    float* ipart;   // symmetric tiles: target-side partials [R][n_pad][3]
+    float* jpart;   // symmetric tiles: reaction partials
 };
+// Workspace of a symmetric tile (floats).
+size_t sym_ipart_floats(int n_pad, int K, int R);
+size_t sym_jpart_floats(int n_pad, int K);
 
 // Reference arithmetic, one fused step (pos_new/vel written), T = float/double.
 hipError_t launch_ref_step_f32(const float4* pos_old, float4* pos_new, float4* vel, int n,

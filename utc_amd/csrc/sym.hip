@@ -1,0 +1,215 @@
+// sym.hip -- all-pairs force evaluating every unordered pair once.
+//
+// The field on body i from body j and the reaction on j from i share r, d^2
+// and y^3 = (|r|^2 + eps2)^{-3/2}: a_i += m_j y^3 r, a_j -= m_i y^3 r.  The
+// one-pass kernels (force_tmpl.h) spend 12 VALU issue slots per ordered pair;
+// here an unordered pair costs 16 (3 sub, 3 fma, 2 mul for y^3, 2 mul for the
+// two mass factors, 6 fma for the two accumulations) plus, per 64-source
+// rotation step, 7 wave-wide DPP moves shared by the K targets of a lane:
+// (16 + 7/K)/2 slots per ordered pair, 8.4 at K = 8.
+//
+// Mapping.  Bodies form wave blocks of 64*K (lane l holds K targets).  Wave w
+// takes (a) its own K blocks of 64 sources, target side only, and (b) the
+// source blocks of wave blocks v = w+1 .. w+NW/2 (mod NW), both sides; for even
+// NW the block half way round is taken by the lower half only, so every
+// unordered pair of wave blocks is visited exactly once.  A block of 64
+// sources sits one per lane and rotates around the wave with DPP wave_ror:1
+// for 64 steps, carrying its reaction accumulator, so after 64 steps every
+// target has met every source and each reaction sum is back in its home lane.
+// R waves split a wave block's source range; their target-side partials and
+// the reaction partials go to HBM and a second kernel adds them per body in a
+// fixed order (deterministic) and applies the integrator epilogue.
+#include "force_tmpl.h"
+
+namespace utcnb {
+
+__device__ __forceinline__ float wave_ror1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C /* wave_ror:1 */,
+                                                      0xF, 0xF, false));
+}
+
+// ipart: [R][n_pad][3] target-side sums; jpart: [NW][NW/2][64K][3] reactions.
+template <int K, bool NEWTON>
+__global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict__ src, int NW, int R,
+                                                        float eps2, float* __restrict__ ipart,
+                                                        float* __restrict__ jpart)
+{
+    const int lane = threadIdx.x & 63;
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (gw >= NW * R)
+        return;  // whole waves only
+    const int w = gw / R, r = gw % R;
+    const int half = NW / 2;
+    const int n_pad = NW * 64 * K;
+    float xi[K], yi[K], zi[K], mi[K], ax[K], ay[K], az[K], cx[K], cy[K], cz[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const float4 p = src[w * 64 * K + k * 64 + lane];
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.w;
+        ax[k] = ay[k] = az[k] = cx[k] = cy[k] = cz[k] = 0.f;
+    }
+    if (r == 0) {
+        // own blocks: target side only (each ordered pair once, self pair gives 0)
+        for (int kb = 0; kb < K; ++kb) {
+            const float4 s = src[w * 64 * K + kb * 64 + lane];
+            float sx = s.x, sy = s.y, sz = s.z, sm = s.w;
+            float px[K], py[K], pz[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                px[k] = py[k] = pz[k] = 0.f;
+#pragma unroll 4
+            for (int t = 0; t < 64; ++t) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+                    const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+                    const float si = pair_scale<NEWTON>(d2, sm);
+                    px[k] = __builtin_fmaf(rx, si, px[k]);
+                    py[k] = __builtin_fmaf(ry, si, py[k]);
+                    pz[k] = __builtin_fmaf(rz, si, pz[k]);
+                }
+                sx = wave_ror1(sx); sy = wave_ror1(sy); sz = wave_ror1(sz); sm = wave_ror1(sm);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                kahan_add(ax[k], cx[k], px[k]);
+                kahan_add(ay[k], cy[k], py[k]);
+                kahan_add(az[k], cz[k], pz[k]);
+            }
+        }
+    }
+    const int d0 = (int)((long long)half * r / R), d1 = (int)((long long)half * (r + 1) / R);
+    for (int d = d0; d < d1; ++d) {
+        if (2 * (d + 1) == NW && w >= half)
+            continue;
+        int v = w + 1 + d;
+        if (v >= NW)
+            v -= NW;
+        for (int kb = 0; kb < K; ++kb) {
+            const float4 s = src[v * 64 * K + kb * 64 + lane];
+            float sx = s.x, sy = s.y, sz = s.z, sm = s.w, gx = 0.f, gy = 0.f, gz = 0.f;
+            float px[K], py[K], pz[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                px[k] = py[k] = pz[k] = 0.f;
+#pragma unroll 4
+            for (int t = 0; t < 64; ++t) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+                    const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+                    const float y3 = pair_scale<NEWTON>(d2, 1.0f);
+                    const float si = sm * y3;
+                    const float cj = mi[k] * y3;
+                    px[k] = __builtin_fmaf(rx, si, px[k]);
+                    py[k] = __builtin_fmaf(ry, si, py[k]);
+                    pz[k] = __builtin_fmaf(rz, si, pz[k]);
+                    gx = __builtin_fmaf(-rx, cj, gx);
+                    gy = __builtin_fmaf(-ry, cj, gy);
+                    gz = __builtin_fmaf(-rz, cj, gz);
+                }
+                sx = wave_ror1(sx); sy = wave_ror1(sy); sz = wave_ror1(sz); sm = wave_ror1(sm);
+                gx = wave_ror1(gx); gy = wave_ror1(gy); gz = wave_ror1(gz);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                kahan_add(ax[k], cx[k], px[k]);
+                kahan_add(ay[k], cy[k], py[k]);
+                kahan_add(az[k], cz[k], pz[k]);
+            }
+            const size_t o = (((size_t)w * half + d) * (64 * K) + kb * 64 + lane) * 3;
+            jpart[o + 0] = gx;
+            jpart[o + 1] = gy;
+            jpart[o + 2] = gz;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const size_t o = ((size_t)r * n_pad + w * 64 * K + k * 64 + lane) * 3;
+        ipart[o + 0] = ax[k] - cx[k];
+        ipart[o + 1] = ay[k] - cy[k];
+        ipart[o + 2] = az[k] - cz[k];
+    }
+}
+
+// Sum of the partials of body b in FP64, in a fixed order, then the epilogue.
+template <class Epi>
+__global__ __launch_bounds__(256) void sym_reduce_kernel(const float* __restrict__ ipart,
+                                                         const float* __restrict__ jpart, int n, int NW,
+                                                         int R, int K, Epi epi)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= n)
+        return;
+    const int n_pad = NW * 64 * K;
+    const int half = NW / 2, v = b / (64 * K), loc = b - v * (64 * K);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int r = 0; r < R; ++r) {
+        const float* p = ipart + ((size_t)r * n_pad + b) * 3;
+        s0 += p[0];
+        s1 += p[1];
+        s2 += p[2];
+    }
+    for (int d = 0; d < half; ++d) {
+        int w = v - 1 - d;
+        if (w < 0)
+            w += NW;
+        if (2 * (d + 1) == NW && w >= half)
+            continue;
+        const float* p = jpart + (((size_t)w * half + d) * (64 * K) + loc) * 3;
+        s0 += p[0];
+        s1 += p[1];
+        s2 += p[2];
+    }
+    // the epilogues take (hi, c) with value hi - c; split the FP64 sum exactly
+    const float h0 = (float)s0, h1 = (float)s1, h2 = (float)s2;
+    epi(b, h0, h1, h2, (float)((double)h0 - s0), (float)((double)h1 - s1), (float)((double)h2 - s2));
+}
+
+size_t sym_ipart_floats(int n_pad, int K, int R) { (void)K; return (size_t)R * n_pad * 3; }
+size_t sym_jpart_floats(int n_pad, int K)
+{
+    const int NW = n_pad / (64 * K);
+    return (size_t)NW * (NW / 2) * 64 * K * 3;
+}
+
+template <class Epi>
+hipError_t launch_sym(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
+                      float* ipart, float* jpart, const Epi& epi, hipStream_t st)
+{
+    const int K = t.k, R = t.s;
+    if (n_pad % (128 * K) != 0 || !ipart || !jpart)
+        return hipErrorInvalidValue;
+    const int NW = n_pad / (64 * K);
+    const int waves = NW * R;
+    const dim3 grid((waves + 3) / 4), block(256);
+    if (K == 4) {
+        if (newton)
+            hipLaunchKernelGGL((sym_force_kernel<4, true>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
+        else
+            hipLaunchKernelGGL((sym_force_kernel<4, false>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
+    } else if (K == 8) {
+        if (newton)
+            hipLaunchKernelGGL((sym_force_kernel<8, true>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
+        else
+            hipLaunchKernelGGL((sym_force_kernel<8, false>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL((sym_reduce_kernel<Epi>), dim3((n + 255) / 256), dim3(256), 0, st, ipart, jpart, n,
+                       NW, R, K, epi);
+    return hipGetLastError();
+}
+
+template hipError_t launch_sym<EpiFP32>(const Tile&, bool, const float4*, int, int, float, float*, float*,
+                                        const EpiFP32&, hipStream_t);
+template hipError_t launch_sym<EpiMixed>(const Tile&, bool, const float4*, int, int, float, float*, float*,
+                                         const EpiMixed&, hipStream_t);
+template hipError_t launch_sym<EpiAccel>(const Tile&, bool, const float4*, int, int, float, float*, float*,
+                                         const EpiAccel&, hipStream_t);
+
+}  // namespace utcnb
