@@ -3,8 +3,10 @@
 
 Workload (default): N = 262144 bodies, Plummer sphere (a = 1, M = 1, equal
 masses, seed 0), eps = 0.002, dt = 2.5e-4, physical kick, K timed steps.  A
-step is one launch of the fused all-pairs force + integrator kernel; the timed
-region covers exactly K steps with the state resident in HBM.
+step is one all-pairs force evaluation with the integrator fused into its last
+kernel (at this size: the symmetric pair kernel, then the kernel that sums its
+partials and applies the FP64 kick-drift); the timed region covers exactly K
+steps with the state resident in HBM.
 
 Prints ONE JSON line (stdout).  `value` is interactions/s of the mixed path
 (FP32 pairs with compensated sums, FP64 state): N^2 * K / wall time.  Also
@@ -181,11 +183,11 @@ def main():
         variants = [v for v in variants if v[0] == args.only]
     for label, prec, flags in variants:
         wall, kms, launches, s = timed_steps(U, prec, n, c, K, W, flags=flags)
-        results[label] = dict(wall_ms=wall, kernel_ms=kms, launches=launches,
+        results[label] = dict(wall_ms=wall, kernel_ms=kms, launches=launches, tile=s.tile()[1],
                               interactions_per_s=pairs * K / (wall * 1e-3),
                               kernel_interactions_per_s=pairs * launches / (kms * 1e-3))
         systems[label] = s
-        log(f"{label}: {wall / K:.3f} ms/step wall, {kms / launches:.3f} ms/step kernel, "
+        log(f"{label} [{s.tile()[1]}]: {wall / K:.3f} ms/step wall, {kms / launches:.3f} ms/step kernel, "
             f"{results[label]['interactions_per_s']:.4e} interactions/s")
 
     if args.only:
@@ -277,7 +279,12 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32,
         "traffic": traffic,
-        "kernel": "force_kernel<K,S,COMP=1> + EpiMixed (fused force, FP64 kick-drift, snapshot)",
+        "kernel": (f"tile {m['tile']}: sym_force_kernel<K> (each unordered pair once, FP32, compensated) "
+                   "+ sym_reduce_kernel<EpiMixed> (FP64 sum of partials, FP64 kick-drift, FP32 snapshot); "
+                   "avg_launch_ms is the two kernels together, one step"
+                   if m["tile"].startswith("sym") else
+                   f"tile {m['tile']}: force_kernel<K,S,W,COMP=1,EpiMixed> (fused force, FP64 kick-drift, "
+                   "FP32 snapshot); avg_launch_ms is one step"),
         "flop_per_launch": FLOP_PER_PAIR * pairs,
         "avg_launch_ms": t_launch * 1e3,
     }

@@ -152,6 +152,11 @@ int utcnb_adjoint(utcnb_system* sys, int nsteps, int ckpt_every, const double* t
  * *launches receives their count.  Synchronizes the stream. */
 int utcnb_last_kernel_ms(utcnb_system* sys, double* ms, int* launches);
 
+/* Force-kernel tile id (1..utcnb_tile_count()) the handle runs: the configured
+ * one, or the built-in default for its size when utcnb_config.tile was 0.
+ * No reference counterpart (the reference's block size is a launch argument). */
+int utcnb_get_tile(const utcnb_system* sys, int* tile);
+
 /* ---- batched systems ---------------------------------------------------- */
 typedef struct utcnb_batch utcnb_batch;
 

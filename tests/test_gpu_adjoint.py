@@ -146,6 +146,55 @@ def test_config4_mixed_adjoint_close_to_fp64(config4_grads):
     assert relnorm(dm, dm64) <= 1e-5
 
 
+def tile_id(name):
+    return U.tiles().index(name) + 1
+
+
+@pytest.mark.parametrize("kick", [U.KICK_PHYSICAL, U.KICK_REFERENCE])
+def test_self_pair_exact_at_tiny_softening(kick):
+    """eps = 1e-6: the pair sums include j = i, where J(0) = eps^-3 I = 1e18 I
+    multiplies w = m_i a_i - m_i a_i.  The kernels must make that w exactly 0
+    (the oracle skips j = i); a rounding residual of m_i a_i there would swamp
+    the gradient.  FP64, and mixed with the one-pass and the symmetric VJP."""
+    n, steps, eps2, dt = 200, 4, 1e-12, 2e-3
+    pos, vel, tgt = small_case(n, steps, seed=5)
+    L0, dx0, dv0, dm0 = O.adjoint_f64(pos, vel, tgt, steps, eps2, dt, 1.0, kick)
+    for prec, tile, tol in ((U.PREC_FP64, 0, 1e-11), (U.PREC_MIXED, tile_id("k1s16"), 1e-5),
+                            (U.PREC_MIXED, tile_id("sym4r32"), 1e-5)):
+        say(f"case: self pair eps=1e-6 kick={kick} prec={prec} tile={tile}")
+        with U.System(n, prec, eps2, dt, kick=kick, tile=tile) as s:
+            s.set_state(pos, vel)
+            L, dx, dv, dm = s.adjoint(steps, 2, tgt)
+        errs = (relnorm(dx, dx0), relnorm(dv, dv0), relnorm(dm, dm0))
+        say(f"  rel err dx {errs[0]:.2e} dv {errs[1]:.2e} dm {errs[2]:.2e}")
+        assert max(errs) <= tol
+
+
+def test_symmetric_vjp_matches_one_pass_and_fp64():
+    """A symmetric force tile also evaluates the adjoint pair sum once per
+    unordered pair (sym_vjp.hip).  N = 5000 (padded to 5120, ragged last wave
+    block), physical kick: gradients of the symmetric VJP against the
+    one-pass VJP and the FP64 adjoint."""
+    n, steps, eps2, dt = 5000, 6, 1e-4, 1e-3
+    pos, vel, tgt = small_case(n, steps, seed=7)
+    res = {}
+    for label, prec, tile in (("fp64", U.PREC_FP64, 0), ("onepass", U.PREC_MIXED, tile_id("k1s16")),
+                              ("sym", U.PREC_MIXED, tile_id("sym4r32")),
+                              ("sym8", U.PREC_MIXED, tile_id("sym8r32"))):
+        say(f"case: VJP {label} n={n}")
+        with U.System(n, prec, eps2, dt, tile=tile) as s:
+            s.set_state(pos, vel)
+            res[label] = s.adjoint(steps, 3, tgt)
+    for label in ("onepass", "sym", "sym8"):
+        e = [relnorm(res[label][q], res["fp64"][q]) for q in (1, 2, 3)]
+        say(f"  {label} vs fp64: dx {e[0]:.2e} dv {e[1]:.2e} dm {e[2]:.2e}")
+        assert max(e) <= 1e-5
+        assert res[label][0] == pytest.approx(res["fp64"][0], rel=1e-9)
+    e = [relnorm(res["sym"][q], res["onepass"][q]) for q in (1, 2, 3)]
+    say(f"  sym vs onepass: dx {e[0]:.2e} dv {e[1]:.2e} dm {e[2]:.2e}")
+    assert max(e) <= 1e-5
+
+
 def test_autograd_function_wrapper():
     """A torch.autograd.Function whose forward and backward run through the C
     ABI (utcnb_adjoint); it lives only in this test harness."""

@@ -30,7 +30,7 @@ GEN_CUBE, GEN_CUBE_LOGM, GEN_PLUMMER, GEN_TARGETS = 0, 1, 2, 3
 EXPORTS = [
     "utcnb_integrate_f32", "utcnb_integrate_f64", "utcnb_create", "utcnb_destroy",
     "utcnb_set_state", "utcnb_get_state", "utcnb_step", "utcnb_sync", "utcnb_accel",
-    "utcnb_diagnostics", "utcnb_adjoint", "utcnb_last_kernel_ms", "utcnb_batch_create",
+    "utcnb_diagnostics", "utcnb_adjoint", "utcnb_last_kernel_ms", "utcnb_get_tile", "utcnb_batch_create",
     "utcnb_batch_destroy", "utcnb_batch_set_state", "utcnb_batch_get_state",
     "utcnb_batch_run", "utcnb_autotune", "utcnb_tile_count", "utcnb_tile_name",
     "utcnb_rng_seed", "utcnb_rng_next", "utcnb_splitmix64", "utcnb_generate",
@@ -77,6 +77,7 @@ def _proto(lib):
         "utcnb_diagnostics": (C.c_int, [_vp, _dp, _dp, _vp]),
         "utcnb_adjoint": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _dp, _vp, _vp, _vp]),
         "utcnb_last_kernel_ms": (C.c_int, [_vp, _dp, P(C.c_int)]),
+        "utcnb_get_tile": (C.c_int, [_vp, P(C.c_int)]),
         "utcnb_batch_create": (C.c_int, [C.c_int, _vp, P(Config), _vp, P(_vp)]),
         "utcnb_batch_destroy": (C.c_int, [_vp]),
         "utcnb_batch_set_state": (C.c_int, [_vp, _vp, _vp]),
@@ -199,6 +200,12 @@ class System:
         ms, cnt = C.c_double(), C.c_int()
         check(lib().utcnb_last_kernel_ms(self.h, C.byref(ms), C.byref(cnt)), "utcnb_last_kernel_ms")
         return ms.value, cnt.value
+
+    def tile(self):
+        """(tile id, tile name) of the force kernel this system runs."""
+        t = C.c_int()
+        check(lib().utcnb_get_tile(self.h, C.byref(t)), "utcnb_get_tile")
+        return t.value, tiles()[t.value - 1]
 
     def adjoint(self, nsteps: int, ckpt_every: int, target3: np.ndarray):
         n = self.n

@@ -52,6 +52,7 @@ LIB_SOURCES = [
     "ref.hip",
     "force_fp32.hip",
     "sym.hip",
+    "sym_vjp.hip",
     "force_fp64.hip",
     "aux.hip",
     "adjoint.hip",

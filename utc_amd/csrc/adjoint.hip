@@ -16,6 +16,7 @@
 // Per pair: 30 VALU ops + one v_rsq (FP32) or a correctly rounded sqrt and
 // division (FP64), with Kahan-compensated 64-source chunks.
 #include "internal.h"
+#include "vjp_pair.h"
 
 namespace utcnb {
 
@@ -55,6 +56,36 @@ __device__ __forceinline__ void kahan(T& s, T& c, T p)
     s = t;
 }
 
+// One chunk of 64 sources into the partial sums p (x, y, z, m) of target i.
+// EXACT: the chunk may hold j = i (vjp_pair.h).
+template <bool EXACT, typename T, typename T4>
+__device__ __forceinline__ void vjp_chunk(const T4* __restrict__ jrec, int jc, const T4& pi,
+                                          const T4& ai, T eps2, T p[4])
+{
+#pragma unroll 2
+    for (int j = jc; j < jc + kChunk; ++j) {
+        const T4 pj = jrec[2 * j];
+        const T4 aj = jrec[2 * j + 1];
+        T rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
+        T d2 = fma_(rx, rx, fma_(ry, ry, fma_(rz, rz, eps2)));
+        T y = inv_sqrt<T>(d2);
+        T y2 = y * y;
+        T y3 = y2 * y;
+        T y5 = y3 * y2;
+        // w = m_i alpha_j - m_j alpha_i
+        T wx = w_term<EXACT>(pi.w, aj.x, pj.w, ai.x);
+        T wy = w_term<EXACT>(pi.w, aj.y, pj.w, ai.y);
+        T wz = w_term<EXACT>(pi.w, aj.z, pj.w, ai.z);
+        T rw = fma_(rx, wx, fma_(ry, wy, rz * wz));
+        T q = (T)(-3) * y5 * rw;
+        p[0] = fma_(q, rx, fma_(y3, wx, p[0]));
+        p[1] = fma_(q, ry, fma_(y3, wy, p[1]));
+        p[2] = fma_(q, rz, fma_(y3, wz, p[2]));
+        T ar = fma_(aj.x, rx, fma_(aj.y, ry, aj.z * rz));
+        p[3] = fma_(-y3, ar, p[3]);
+    }
+}
+
 // jrec: per body two T4 records {x,y,z,m} {ax,ay,az,0}; n_pad bodies (padding:
 // zero mass, zero alpha, far away).  xbar (double4 per body: x,y,z,-) and mbar
 // are accumulated in place for the first n bodies.
@@ -77,30 +108,13 @@ __global__ __launch_bounds__(64 * W) void vjp_kernel(const typename Vec4<T>::typ
     T g[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
     const int len = n_pad / S;
     const int j0 = part * len;
+    const int jdiag = tile * 64;  // the chunk holding j = i for every lane of the wave
     for (int jc = j0; jc < j0 + len; jc += kChunk) {
         T p[4] = {0, 0, 0, 0};
-#pragma unroll 2
-        for (int j = jc; j < jc + kChunk; ++j) {
-            const T4 pj = jrec[2 * j];
-            const T4 aj = jrec[2 * j + 1];
-            T rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-            T d2 = fma_(rx, rx, fma_(ry, ry, fma_(rz, rz, eps2)));
-            T y = inv_sqrt<T>(d2);
-            T y2 = y * y;
-            T y3 = y2 * y;
-            T y5 = y3 * y2;
-            // w = m_i alpha_j - m_j alpha_i
-            T wx = fma_(-pj.w, ai.x, pi.w * aj.x);
-            T wy = fma_(-pj.w, ai.y, pi.w * aj.y);
-            T wz = fma_(-pj.w, ai.z, pi.w * aj.z);
-            T rw = fma_(rx, wx, fma_(ry, wy, rz * wz));
-            T q = (T)(-3) * y5 * rw;
-            p[0] = fma_(q, rx, fma_(y3, wx, p[0]));
-            p[1] = fma_(q, ry, fma_(y3, wy, p[1]));
-            p[2] = fma_(q, rz, fma_(y3, wz, p[2]));
-            T ar = fma_(aj.x, rx, fma_(aj.y, ry, aj.z * rz));
-            p[3] = fma_(-y3, ar, p[3]);
-        }
+        if (jc == jdiag)
+            vjp_chunk<true>(jrec, jc, pi, ai, eps2, p);
+        else
+            vjp_chunk<false>(jrec, jc, pi, ai, eps2, p);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             kahan(g[q], c[q], p[q]);

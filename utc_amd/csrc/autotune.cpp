@@ -104,8 +104,11 @@ extern "C" int utcnb_autotune(int n, int precision, const char* cache_path, int*
         c.damping = 1.0;
         c.dt = 1e-4;
         utcnb_system* s = nullptr;
-        if ((rc = utcnb_create(&c, nullptr, &s)))
+        if ((rc = utcnb_create(&c, nullptr, &s))) {
+            if (rc == UTCNB_EINVAL && tile_get(t - 1).sym)
+                continue;  // reaction workspace too large at this size: not a candidate
             return rc;
+        }
         double ms = 0.0;
         int launches = 0;
         rc = utcnb_set_state(s, pos.data(), vel.data(), UTCNB_F64, UTCNB_HOST);

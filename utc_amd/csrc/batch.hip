@@ -15,6 +15,7 @@
 //     the same item lists; the whole forward (+ adjoint) sequence is captured
 //     once in a HIP graph and replayed.
 #include "force_tmpl.h"
+#include "vjp_pair.h"
 
 namespace utcnb {
 
@@ -61,6 +62,35 @@ __global__ __launch_bounds__(64 * W) void batch_step_kernel(BatchGeom g, const f
     }
 }
 
+// One chunk of 64 sources of the VJP (same algebra as vjp_chunk in adjoint.hip).
+// EXACT: the chunk may hold j = i (vjp_pair.h).
+template <bool EXACT>
+__device__ __forceinline__ void batch_vjp_chunk(const float4* __restrict__ src, int jc, const float4& pi,
+                                                const float4& ai, float eps2, float p[4])
+{
+#pragma unroll 2
+    for (int j = jc; j < jc + kChunk; ++j) {
+        const float4 pj = src[2 * j];
+        const float4 aj = src[2 * j + 1];
+        float rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
+        float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+        float y = __builtin_amdgcn_rsqf(d2);
+        float y2 = y * y;
+        float y3 = y2 * y;
+        float y5 = y3 * y2;
+        float wx = w_term<EXACT>(pi.w, aj.x, pj.w, ai.x);
+        float wy = w_term<EXACT>(pi.w, aj.y, pj.w, ai.y);
+        float wz = w_term<EXACT>(pi.w, aj.z, pj.w, ai.z);
+        float rw = __builtin_fmaf(rx, wx, __builtin_fmaf(ry, wy, rz * wz));
+        float q = -3.0f * y5 * rw;
+        p[0] = __builtin_fmaf(q, rx, __builtin_fmaf(y3, wx, p[0]));
+        p[1] = __builtin_fmaf(q, ry, __builtin_fmaf(y3, wy, p[1]));
+        p[2] = __builtin_fmaf(q, rz, __builtin_fmaf(y3, wz, p[2]));
+        float ar = __builtin_fmaf(aj.x, rx, __builtin_fmaf(aj.y, ry, aj.z * rz));
+        p[3] = __builtin_fmaf(-y3, ar, p[3]);
+    }
+}
+
 // Adjoint VJP over items (same algebra as vjp_kernel in adjoint.hip).
 template <int W>
 __global__ __launch_bounds__(64 * W) void batch_vjp_kernel(BatchGeom g, const float4* __restrict__ jrec,
@@ -83,27 +113,10 @@ __global__ __launch_bounds__(64 * W) void batch_vjp_kernel(BatchGeom g, const fl
         const float4* src = jrec + 2 * (size_t)off;
         for (int jc = c0 * kChunk; jc < c1 * kChunk; jc += kChunk) {
             float p[4] = {0, 0, 0, 0};
-#pragma unroll 2
-            for (int j = jc; j < jc + kChunk; ++j) {
-                const float4 pj = src[2 * j];
-                const float4 aj = src[2 * j + 1];
-                float rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-                float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-                float y = __builtin_amdgcn_rsqf(d2);
-                float y2 = y * y;
-                float y3 = y2 * y;
-                float y5 = y3 * y2;
-                float wx = __builtin_fmaf(-pj.w, ai.x, pi.w * aj.x);
-                float wy = __builtin_fmaf(-pj.w, ai.y, pi.w * aj.y);
-                float wz = __builtin_fmaf(-pj.w, ai.z, pi.w * aj.z);
-                float rw = __builtin_fmaf(rx, wx, __builtin_fmaf(ry, wy, rz * wz));
-                float q = -3.0f * y5 * rw;
-                p[0] = __builtin_fmaf(q, rx, __builtin_fmaf(y3, wx, p[0]));
-                p[1] = __builtin_fmaf(q, ry, __builtin_fmaf(y3, wy, p[1]));
-                p[2] = __builtin_fmaf(q, rz, __builtin_fmaf(y3, wz, p[2]));
-                float ar = __builtin_fmaf(aj.x, rx, __builtin_fmaf(aj.y, ry, aj.z * rz));
-                p[3] = __builtin_fmaf(-y3, ar, p[3]);
-            }
+            if (jc == w.y * 64)  // the chunk holding j = i for every lane
+                batch_vjp_chunk<true>(src, jc, pi, ai, eps2, p);
+            else
+                batch_vjp_chunk<false>(src, jc, pi, ai, eps2, p);
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 kahan_add(gs[q], cs[q], p[q]);

@@ -70,6 +70,9 @@ struct utcnb_system {
         double* mbar = nullptr;      // n
         double* tgt = nullptr;       // 3n
         double* part = nullptr;      // loss partials + loss
+        int vjp_r = 0;               // symmetric VJP: waves per wave block (0 = one-pass VJP)
+        float4* vjp_ipart = nullptr; // symmetric VJP partials
+        float4* vjp_jpart = nullptr;
     } adj;
     bool timed = false;
     int cu_count = 256;
@@ -136,7 +139,9 @@ void release_adjoint(utcnb_system* s)
     dfree(a.mbar);
     dfree(a.tgt);
     dfree(a.part);
-    a.nck = a.seg = 0;
+    dfree(a.vjp_ipart);
+    dfree(a.vjp_jpart);
+    a.nck = a.seg = a.vjp_r = 0;
 }
 
 void release(utcnb_system* s)
@@ -553,6 +558,14 @@ int utcnb_sync(utcnb_system* s)
     return UTCNB_OK;
 }
 
+int utcnb_get_tile(const utcnb_system* s, int* tile)
+{
+    if (!s || !tile)
+        return fail(UTCNB_EINVAL, "utcnb_get_tile: null argument");
+    *tile = s->args.tile + 1;
+    return UTCNB_OK;
+}
+
 int utcnb_last_kernel_ms(utcnb_system* s, double* ms, int* launches)
 {
     if (!s || !ms)
@@ -617,6 +630,14 @@ static int adjoint_workspace(utcnb_system* s, int nck, int seg)
              dalloc(&a.tgt, 3 * n) || dalloc(&a.part, (n + 255) / 256 + 1);
     if (!rc && f32)
         rc = dalloc(&a.wsnap[0], np) || dalloc(&a.wsnap[1], np);
+    if (!rc && f32 && tile_get(s->args.tile).sym && sym_vjp_applies((int)np) &&
+        sym_vjp_jpart_float4s((int)np) * sizeof(float4) <= ((size_t)16 << 30)) {
+        // a symmetric force tile also evaluates the adjoint pair sum symmetrically
+        const int R = sym_vjp_splits((int)np, s->cu_count);
+        rc = dalloc(&a.vjp_ipart, sym_vjp_ipart_float4s((int)np, R)) ||
+             dalloc(&a.vjp_jpart, sym_vjp_jpart_float4s((int)np));
+        a.vjp_r = R;
+    }
     if (!rc) {
         const size_t rb = f32 ? sizeof(float4) : sizeof(double4);
         if (hipMalloc(&a.jrec, 2 * np * rb) != hipSuccess || hipMalloc(&a.fsrc, np * rb) != hipSuccess)
@@ -741,7 +762,10 @@ int utcnb_adjoint(utcnb_system* s, int nsteps, int ckpt_every, const double* tar
             }
             ADJ_CHECK(launch_adjoint_pre(mixed, xs, acc, s->n, s->cfg.dt, s->cfg.damping,
                                          s->cfg.kick, a.xbar, a.vbar, a.mbar, a.jrec, st));
-            if (mixed)
+            if (mixed && a.vjp_r > 0)
+                ADJ_CHECK(launch_sym_vjp((const float4*)a.jrec, s->n, s->n_pad, s->args.eps2f, a.vjp_r,
+                                         a.vjp_ipart, a.vjp_jpart, a.xbar, a.mbar, st));
+            else if (mixed)
                 ADJ_CHECK(launch_vjp_f32((const float4*)a.jrec, s->n, s->n_pad, s->cfg.eps2,
                                          a.xbar, a.mbar, s->cu_count, st));
             else

@@ -12,23 +12,39 @@ const Tile kTiles[] = {
     {1, 16, "k1s16", false},
     // symmetric (sym.hip): K targets per lane, R waves per wave block
     {4, 4, "sym4r4", true}, {4, 8, "sym4r8", true}, {8, 4, "sym8r4", true},
-    {8, 8, "sym8r8", true}, {8, 16, "sym8r16", true},
+    {8, 8, "sym8r8", true}, {8, 16, "sym8r16", true}, {4, 16, "sym4r16", true},
+    {4, 32, "sym4r32", true}, {8, 32, "sym8r32", true},
 };
 }  // namespace
 
 int tile_count() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 const Tile& tile_get(int id) { return kTiles[id]; }
 
-// Default tile.  The source loop hides its scalar-load latency only with
-// several waves per SIMD, so the default splits the sources until the grid
-// holds ~16 waves per SIMD (measured on MI355X: at N = 262144, k2s8 runs at
-// 20.5 ms/step vs 27.2 ms for k2s1; at N <= 16384, k1s16 is fastest).
+// Default tile, from sweeps of every tile on MI355X (mixed precision, ms/step):
+//   N = 262144: sym8r32 14.33, sym8r16 14.54, k2s8 20.7 (best one-pass)
+//   N = 131072: sym8r32 3.73,  k2s8 5.30
+//   N =  65536: sym4r32 1.058, sym8r32 1.066, k2s8 1.40
+//   N =  32768: sym4r32 0.320, k2s8 0.426
+//   N =  16384: sym4r32 0.115, k1s16 0.117
+//   N =   8192: k1s16 0.061,  sym4r32 0.062
+// The symmetric tiles need enough wave blocks to fill the chip (K = 8 from 512
+// bodies per CU, K = 4 from 64), and their reaction partials must fit the
+// 16 GiB workspace limit; above it the best one-pass tile (k2s8) is used.
 int tile_default(int64_t n, int cu_count)
 {
-    const int64_t simds = (int64_t)(cu_count > 0 ? cu_count : 256) * 4;
-    if ((n + 127) / 128 * 8 >= simds * 4)
-        return 8;  // k2s8: N >= 65536 on 256 CUs
-    return 9;      // k1s16
+    const int64_t cus = cu_count > 0 ? cu_count : 256;
+    const int64_t n_pad = pad_j(n);
+    const auto fits = [&](int k) {
+        return n_pad <= (1 << 30) && n_pad % (128 * k) == 0 &&
+               sym_jpart_floats((int)n_pad, k) * sizeof(float) <= ((size_t)16 << 30);
+    };
+    if (n >= 512 * cus && fits(8))
+        return 17;  // sym8r32
+    if (n >= 64 * cus && fits(4))
+        return 16;  // sym4r32
+    if ((n + 127) / 128 * 8 >= cus * 16)
+        return 8;   // k2s8
+    return 9;       // k1s16
 }
 
 template <class Epi>

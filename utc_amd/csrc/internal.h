@@ -103,6 +103,15 @@ hipError_t launch_adjoint_pre(bool fp32_records, const double4* pos, const doubl
                               double* mbar, void* jrec, hipStream_t st);
 hipError_t launch_jrec_pad(bool fp32_records, void* jrec, int n, int n_pad, hipStream_t st);
 
+// ---- symmetric adjoint pair sum (sym_vjp.hip): FP32 records, K = 4 ----------
+bool sym_vjp_applies(int n_pad);                 // n_pad a multiple of 512
+int sym_vjp_splits(int n_pad, int cu_count);     // waves per wave block
+size_t sym_vjp_ipart_float4s(int n_pad, int R);
+size_t sym_vjp_jpart_float4s(int n_pad);
+// xbar3 / mbar (first n bodies) += the pair sums; ipart / jpart are workspace.
+hipError_t launch_sym_vjp(const float4* jrec, int n, int n_pad, float eps2, int R, float4* ipart,
+                          float4* jpart, double* xbar3, double* mbar, hipStream_t st);
+
 // ---- batched systems (batch.hip) ---------------------------------------------
 struct BatchGeom {
     const int* off;    // first body of each system (device)

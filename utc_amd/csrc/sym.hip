@@ -16,8 +16,10 @@
 // sources sits one per lane and rotates around the wave with DPP wave_ror:1
 // for 64 steps, carrying its reaction accumulator, so after 64 steps every
 // target has met every source and each reaction sum is back in its home lane.
-// R waves split a wave block's source range; their target-side partials and
-// the reaction partials go to HBM and a second kernel adds them per body in a
+// R waves split a wave block's work: unit 0 is the own blocks, unit 1 + d the
+// wave block d + 1 places round; wave r of the block takes units
+// [U*r/R, U*(r+1)/R) of U = NW/2 + 1.  Their target-side partials and the
+// reaction partials go to HBM and a second kernel adds them per body in a
 // fixed order (deterministic) and applies the integrator epilogue.
 #include "force_tmpl.h"
 
@@ -49,7 +51,9 @@ __global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict
         xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.w;
         ax[k] = ay[k] = az[k] = cx[k] = cy[k] = cz[k] = 0.f;
     }
-    if (r == 0) {
+    const int units = half + 1;
+    const int u0 = (int)((long long)units * r / R), u1 = (int)((long long)units * (r + 1) / R);
+    if (u0 == 0 && u1 > 0) {
         // own blocks: target side only (each ordered pair once, self pair gives 0)
         for (int kb = 0; kb < K; ++kb) {
             const float4 s = src[w * 64 * K + kb * 64 + lane];
@@ -79,7 +83,7 @@ __global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict
             }
         }
     }
-    const int d0 = (int)((long long)half * r / R), d1 = (int)((long long)half * (r + 1) / R);
+    const int d0 = u0 > 0 ? u0 - 1 : 0, d1 = u1 - 1;
     for (int d = d0; d < d1; ++d) {
         if (2 * (d + 1) == NW && w >= half)
             continue;

@@ -1,0 +1,238 @@
+// sym_vjp.hip -- the adjoint pair sum (VJP) evaluating every unordered pair once.
+//
+// With alpha the per-body kick adjoint, r = x_j - x_i, d^2 = |r|^2 + eps2,
+// y = d^-1, J(r) = y^3 I - 3 y^5 r r^T and w = m_i alpha_j - m_j alpha_i, the
+// pair (i, j) adds
+//   xb_i += J w,            xb_j -= J w            (J is even in r, w odd in i<->j)
+//   mb_i -= y^3 alpha_j.r,  mb_j += y^3 alpha_i.r
+// so both bodies' terms come from one evaluation: 41 VALU slots per unordered
+// pair (3 sub, 3 fma, rsq, 4 mul for y^3 and -3y^5, 6 for w, 3 for r.w, 1 mul,
+// 12 fma for the two position terms, 6 for the two dot products, 2 fma for the
+// two mass terms) against 31 per ordered pair in the one-pass VJP, plus 11 DPP
+// moves per 64-source rotation step shared by the K targets of a lane.
+//
+// The mapping is the symmetric force kernel's (sym.hip): wave blocks of 64*K
+// bodies, unit 0 = own blocks (target side only), unit 1 + d = the wave block
+// d + 1 places round (both sides), R waves per wave block split the units.
+// Partials go to HBM (float4 per body: x, y, z, m) and a second kernel adds
+// them per body in FP64, in a fixed order, into xbar and mbar.
+#include "internal.h"
+#include "vjp_pair.h"
+
+namespace utcnb {
+
+namespace {
+
+__device__ __forceinline__ float ror1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C /* wave_ror:1 */,
+                                                      0xF, 0xF, false));
+}
+
+__device__ __forceinline__ void kahan1(float& s, float& c, float p)
+{
+    const float y = p - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+}  // namespace
+
+// jrec: two float4 per body {x,y,z,m} {ax,ay,az,0}, n_pad = NW*64*K bodies.
+// ipart: [R][n_pad] float4; jpart: [NW][NW/2][64K] float4.
+template <int K>
+__global__ __launch_bounds__(256) void sym_vjp_kernel(const float4* __restrict__ jrec, int NW, int R,
+                                                      float eps2, float4* __restrict__ ipart,
+                                                      float4* __restrict__ jpart)
+{
+    const int lane = threadIdx.x & 63;
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (gw >= NW * R)
+        return;  // whole waves only
+    const int w = gw / R, r = gw % R;
+    const int half = NW / 2;
+    const int n_pad = NW * 64 * K;
+    float xi[K], yi[K], zi[K], mi[K], bx[K], by[K], bz[K];
+    float g[K][4], c[K][4];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int b = w * 64 * K + k * 64 + lane;
+        const float4 p = jrec[2 * b], a = jrec[2 * b + 1];
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.w;
+        bx[k] = a.x; by[k] = a.y; bz[k] = a.z;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            g[k][q] = c[k][q] = 0.f;
+    }
+    const int units = half + 1;
+    const int u0 = (int)((long long)units * r / R), u1 = (int)((long long)units * (r + 1) / R);
+    if (u0 == 0 && u1 > 0) {
+        // own blocks: target side only; they hold the self pairs, so w is exact (vjp_pair.h)
+        for (int kb = 0; kb < K; ++kb) {
+            const int b = w * 64 * K + kb * 64 + lane;
+            const float4 sp = jrec[2 * b], sa = jrec[2 * b + 1];
+            float sx = sp.x, sy = sp.y, sz = sp.z, sm = sp.w, ax = sa.x, ay = sa.y, az = sa.z;
+            float p[K][4];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
+#pragma unroll 2
+            for (int t = 0; t < 64; ++t) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+                    const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+                    const float y = __builtin_amdgcn_rsqf(d2);
+                    const float y2 = y * y, y3 = y2 * y, y5m3 = (-3.f * y2) * y3;
+                    const float wx = w_exact(mi[k], ax, sm, bx[k]);
+                    const float wy = w_exact(mi[k], ay, sm, by[k]);
+                    const float wz = w_exact(mi[k], az, sm, bz[k]);
+                    const float q = y5m3 * __builtin_fmaf(rx, wx, __builtin_fmaf(ry, wy, rz * wz));
+                    p[k][0] = __builtin_fmaf(q, rx, __builtin_fmaf(y3, wx, p[k][0]));
+                    p[k][1] = __builtin_fmaf(q, ry, __builtin_fmaf(y3, wy, p[k][1]));
+                    p[k][2] = __builtin_fmaf(q, rz, __builtin_fmaf(y3, wz, p[k][2]));
+                    const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
+                    p[k][3] = __builtin_fmaf(-y3, arj, p[k][3]);
+                }
+                sx = ror1(sx); sy = ror1(sy); sz = ror1(sz); sm = ror1(sm);
+                ax = ror1(ax); ay = ror1(ay); az = ror1(az);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    kahan1(g[k][q], c[k][q], p[k][q]);
+        }
+    }
+    const int d0 = u0 > 0 ? u0 - 1 : 0, d1 = u1 - 1;
+    for (int d = d0; d < d1; ++d) {
+        if (2 * (d + 1) == NW && w >= half)
+            continue;
+        int v = w + 1 + d;
+        if (v >= NW)
+            v -= NW;
+        for (int kb = 0; kb < K; ++kb) {
+            const int b = v * 64 * K + kb * 64 + lane;
+            const float4 sp = jrec[2 * b], sa = jrec[2 * b + 1];
+            float sx = sp.x, sy = sp.y, sz = sp.z, sm = sp.w, ax = sa.x, ay = sa.y, az = sa.z;
+            float hx = 0.f, hy = 0.f, hz = 0.f, hm = 0.f;  // reaction on the source body
+            float p[K][4];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
+#pragma unroll 2
+            for (int t = 0; t < 64; ++t) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+                    const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+                    const float y = __builtin_amdgcn_rsqf(d2);
+                    const float y2 = y * y, y3 = y2 * y, y5m3 = (-3.f * y2) * y3;
+                    const float wx = w_fast(mi[k], ax, sm, bx[k]);
+                    const float wy = w_fast(mi[k], ay, sm, by[k]);
+                    const float wz = w_fast(mi[k], az, sm, bz[k]);
+                    const float q = y5m3 * __builtin_fmaf(rx, wx, __builtin_fmaf(ry, wy, rz * wz));
+                    p[k][0] = __builtin_fmaf(q, rx, __builtin_fmaf(y3, wx, p[k][0]));
+                    p[k][1] = __builtin_fmaf(q, ry, __builtin_fmaf(y3, wy, p[k][1]));
+                    p[k][2] = __builtin_fmaf(q, rz, __builtin_fmaf(y3, wz, p[k][2]));
+                    hx = __builtin_fmaf(-q, rx, __builtin_fmaf(-y3, wx, hx));
+                    hy = __builtin_fmaf(-q, ry, __builtin_fmaf(-y3, wy, hy));
+                    hz = __builtin_fmaf(-q, rz, __builtin_fmaf(-y3, wz, hz));
+                    const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
+                    const float ari = __builtin_fmaf(bx[k], rx, __builtin_fmaf(by[k], ry, bz[k] * rz));
+                    p[k][3] = __builtin_fmaf(-y3, arj, p[k][3]);
+                    hm = __builtin_fmaf(y3, ari, hm);
+                }
+                sx = ror1(sx); sy = ror1(sy); sz = ror1(sz); sm = ror1(sm);
+                ax = ror1(ax); ay = ror1(ay); az = ror1(az);
+                hx = ror1(hx); hy = ror1(hy); hz = ror1(hz); hm = ror1(hm);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    kahan1(g[k][q], c[k][q], p[k][q]);
+            jpart[((size_t)w * half + d) * (64 * K) + kb * 64 + lane] = make_float4(hx, hy, hz, hm);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        ipart[(size_t)r * n_pad + w * 64 * K + k * 64 + lane] =
+            make_float4(g[k][0] - c[k][0], g[k][1] - c[k][1], g[k][2] - c[k][2], g[k][3] - c[k][3]);
+}
+
+// xbar_b += sum of body b's partials, mbar_b likewise; FP64, fixed order.
+__global__ __launch_bounds__(256) void sym_vjp_reduce_kernel(const float4* __restrict__ ipart,
+                                                             const float4* __restrict__ jpart, int n,
+                                                             int NW, int R, int K,
+                                                             double* __restrict__ xbar3,
+                                                             double* __restrict__ mbar)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= n)
+        return;
+    const int n_pad = NW * 64 * K;
+    const int half = NW / 2, v = b / (64 * K), loc = b - v * (64 * K);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int r = 0; r < R; ++r) {
+        const float4 p = ipart[(size_t)r * n_pad + b];
+        s0 += p.x; s1 += p.y; s2 += p.z; s3 += p.w;
+    }
+    for (int d = 0; d < half; ++d) {
+        int w = v - 1 - d;
+        if (w < 0)
+            w += NW;
+        if (2 * (d + 1) == NW && w >= half)
+            continue;
+        const float4 p = jpart[((size_t)w * half + d) * (64 * K) + loc];
+        s0 += p.x; s1 += p.y; s2 += p.z; s3 += p.w;
+    }
+    xbar3[3 * (size_t)b + 0] += s0;
+    xbar3[3 * (size_t)b + 1] += s1;
+    xbar3[3 * (size_t)b + 2] += s2;
+    mbar[b] += s3;
+}
+
+constexpr int kSymVjpK = 4;
+
+int sym_vjp_splits(int n_pad, int cu_count)
+{
+    // ~16 waves per SIMD, at most one unit per wave, at most 32 splits
+    const int NW = n_pad / (64 * kSymVjpK);
+    const int want = (cu_count > 0 ? cu_count : 256) * 4 * 16;
+    int R = (want + NW - 1) / NW;
+    const int units = NW / 2 + 1;
+    if (R > units)
+        R = units;
+    if (R > 32)
+        R = 32;
+    return R < 1 ? 1 : R;
+}
+
+size_t sym_vjp_ipart_float4s(int n_pad, int R) { return (size_t)R * n_pad; }
+size_t sym_vjp_jpart_float4s(int n_pad)
+{
+    const int NW = n_pad / (64 * kSymVjpK);
+    return (size_t)NW * (NW / 2) * 64 * kSymVjpK;
+}
+bool sym_vjp_applies(int n_pad) { return n_pad % (128 * kSymVjpK) == 0; }
+
+hipError_t launch_sym_vjp(const float4* jrec, int n, int n_pad, float eps2, int R, float4* ipart,
+                          float4* jpart, double* xbar3, double* mbar, hipStream_t st)
+{
+    if (!sym_vjp_applies(n_pad) || R < 1 || !ipart || !jpart)
+        return hipErrorInvalidValue;
+    const int NW = n_pad / (64 * kSymVjpK);
+    const int waves = NW * R;
+    hipLaunchKernelGGL((sym_vjp_kernel<kSymVjpK>), dim3((waves + 3) / 4), dim3(256), 0, st, jrec, NW, R,
+                       eps2, ipart, jpart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(sym_vjp_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ipart, jpart, n,
+                       NW, R, kSymVjpK, xbar3, mbar);
+    return hipGetLastError();
+}
+
+}  // namespace utcnb
