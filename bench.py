@@ -131,6 +131,36 @@ def batched_config5(U, nsys=256, steps=20, seed=0):
     return out
 
 
+def fwd_adjoint(U, c, n, K, fwd_rate, ckpt=10):
+    """Forward K steps with checkpoints every `ckpt`, then the adjoint of the
+    mean squared distance to seeded targets back to the initial state (mixed
+    precision, default tile).  Rates: forward-equivalent N^2 K / wall, and all
+    pair evaluations (forward, recompute, VJP, each N^2) / wall."""
+    pairs = float(n) * float(n)
+    tgt = U.generate(U.GEN_TARGETS, n, 0x5EED7A56E7, 0.5)
+    sa = U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON)
+    sa.set_state(c["pos"], c["vel"])
+    sa.adjoint(K, ckpt, tgt)  # warm-up with the timed shape (allocates the workspace)
+    sa.set_state(c["pos"], c["vel"])
+    sa.sync()
+    t0 = time.perf_counter()
+    loss, dx, dv, dm = sa.adjoint(K, ckpt, tgt)
+    wall = (time.perf_counter() - t0) * 1e3
+    kms, launches = sa.last_kernel_ms()
+    fe = pairs * K / (wall * 1e-3)
+    out = dict(wall_ms=wall, ckpt_every=ckpt, loss=loss, tile=sa.tile()[1],
+               forward_equivalent_interactions_per_s=fe,
+               pair_evaluations=launches,
+               total_pair_evaluations_per_s=pairs * launches / (wall * 1e-3))
+    if fwd_rate:
+        out["ratio_vs_forward"] = fe / fwd_rate
+        out["total_ratio_vs_forward"] = pairs * launches / (wall * 1e-3) / fwd_rate
+    log(f"fwd+adjoint (ckpt {ckpt}): {wall:.1f} ms, forward-equivalent {fe:.4e}/s "
+        + (f"({fe / fwd_rate:.3f}x forward), " if fwd_rate else "")
+        + f"total pair evaluations {launches}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,7 +171,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8192)
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp32"], default=None,
+    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint"], default=None,
                     help="time one path only and print its line (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 256 MiB device copy (to scale HBM counters) and exit")
@@ -173,6 +203,12 @@ def main():
     log(f"device {name} CUs={cus} clock={clk} kHz; n={n} steps={K} warmup={W}")
 
     pairs = float(n) * float(n)
+    if args.only == "adjoint":
+        r = fwd_adjoint(U, c, n, K, None)
+        print(json.dumps({"metric": METRIC, "value": r["forward_equivalent_interactions_per_s"],
+                          "unit": "interactions/s", "n_gpus": 1, "steps": K, "warmup": W,
+                          "only": "adjoint", "fwd_adjoint": r}), flush=True)
+        return
     results = {}
     systems = {}
     variants = (("mixed", U.PREC_MIXED, U.FLAG_NO_NEWTON),
@@ -232,28 +268,12 @@ def main():
 
     # ---- forward + adjoint ----------------------------------------------------------
     fwd_adj = None
-    if not args.no_adjoint and hasattr(U.lib(), "utcnb_adjoint"):
+    if not args.no_adjoint:
         try:
-            tgt = U.generate(U.GEN_TARGETS, n, 0x5EED7A56E7, 0.5)
-            sa = U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON)
-            sa.set_state(c["pos"], c["vel"])
-            sa.adjoint(K, 10, tgt)  # warm-up with the timed shape (allocates the workspace)
-            sa.set_state(c["pos"], c["vel"])
-            sa.sync()
-            t0 = time.perf_counter()
-            loss, dx, dv, dm = sa.adjoint(K, 10, tgt)
-            wall = (time.perf_counter() - t0) * 1e3
-            kms, launches = sa.last_kernel_ms()
-            fwd_rate = results["mixed"]["interactions_per_s"]
-            fe = pairs * K / (wall * 1e-3)
-            fwd_adj = dict(wall_ms=wall, ckpt_every=10, loss=loss,
-                           forward_equivalent_interactions_per_s=fe,
-                           ratio_vs_forward=fe / fwd_rate,
-                           pair_evaluations=launches,
-                           total_pair_evaluations_per_s=pairs * launches / (wall * 1e-3),
-                           total_ratio_vs_forward=pairs * launches / (wall * 1e-3) / fwd_rate)
-            log(f"fwd+adjoint: {wall:.1f} ms, forward-equivalent {fe:.4e}/s "
-                f"({fe / fwd_rate:.3f}x forward), total pair evaluations {launches}")
+            fwd_adj = fwd_adjoint(U, c, n, K, results["mixed"]["interactions_per_s"])
+            # every step stored (no recompute): the cost of the reverse sweep alone
+            fwd_adj["store_every_step"] = fwd_adjoint(U, c, n, K, results["mixed"]["interactions_per_s"],
+                                                      ckpt=1)
         except Exception as e:  # reported, never hidden
             fwd_adj = {"error": str(e)}
             log(f"fwd+adjoint failed: {e}")

@@ -82,7 +82,10 @@ def _compile(src: Path, obj: Path) -> Path:
 def build_library(jobs: int | None = None) -> Path:
     OUT.mkdir(parents=True, exist_ok=True)
     OBJ.mkdir(parents=True, exist_ok=True)
-    srcs = [CSRC / s for s in LIB_SOURCES if (CSRC / s).exists()]
+    srcs = [CSRC / s for s in LIB_SOURCES]
+    missing = [str(s) for s in srcs if not s.exists()]
+    if missing:
+        raise RuntimeError(f"library sources missing: {missing}")
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, OBJ / (s.name + ".o")), srcs))

@@ -25,10 +25,14 @@
 
 namespace utcnb {
 
+// Lane l receives lane l-1's value (lane 0 lane 63's).  Every lane has a
+// source under wave_ror:1, so the "old" operand (0) is never used.  (Passing
+// v as "old" rotates in place and saves a v_mov per rotation, but ran 0.4%
+// slower on MI355X in a same-device A/B of the N = 262144 step.)
 __device__ __forceinline__ float wave_ror1(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C /* wave_ror:1 */,
-                                                      0xF, 0xF, false));
+    const int x = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x13C /* wave_ror:1 */, 0xF, 0xF, false));
 }
 
 // ipart: [R][n_pad][3] target-side sums; jpart: [NW][NW/2][64K][3] reactions.

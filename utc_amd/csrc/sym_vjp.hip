@@ -5,11 +5,14 @@
 // pair (i, j) adds
 //   xb_i += J w,            xb_j -= J w            (J is even in r, w odd in i<->j)
 //   mb_i -= y^3 alpha_j.r,  mb_j += y^3 alpha_i.r
-// so both bodies' terms come from one evaluation: 41 VALU slots per unordered
-// pair (3 sub, 3 fma, rsq, 4 mul for y^3 and -3y^5, 6 for w, 3 for r.w, 1 mul,
-// 12 fma for the two position terms, 6 for the two dot products, 2 fma for the
-// two mass terms) against 31 per ordered pair in the one-pass VJP, plus 11 DPP
-// moves per 64-source rotation step shared by the K targets of a lane.
+// so both bodies' terms come from one evaluation.  Written as
+// J w = y^3 (w - 3 y^2 (r.w) r) with r.w = m_i (a_j.r) - m_j (a_i.r), reusing
+// the two dot products the mass terms need, an unordered pair costs 36 VALU
+// slots (3 sub, 3 fma for d^2, rsq, 2 mul for y^2 and y^3, 6 for the two dot
+// products, 2 for r.w, 2 mul for -3 y^2 r.w, 6 for w, 3 fma for the bracket,
+// 6 fma for the two position terms, 2 fma for the two mass terms) against 31
+// per ordered pair in the one-pass VJP, plus 11 DPP moves per 64-source
+// rotation step shared by the K targets of a lane.
 //
 // The mapping is the symmetric force kernel's (sym.hip): wave blocks of 64*K
 // bodies, unit 0 = own blocks (target side only), unit 1 + d = the wave block
@@ -23,10 +26,14 @@ namespace utcnb {
 
 namespace {
 
+// Lane l receives lane l-1's value (lane 0 lane 63's).  Every lane has a
+// source under wave_ror:1, so the "old" operand (0) is never used.  (Passing
+// v as "old" rotates in place and saves a v_mov per rotation, but ran 0.4%
+// slower on MI355X in a same-device A/B of the N = 262144 step.)
 __device__ __forceinline__ float ror1(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C /* wave_ror:1 */,
-                                                      0xF, 0xF, false));
+    const int x = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x13C /* wave_ror:1 */, 0xF, 0xF, false));
 }
 
 __device__ __forceinline__ void kahan1(float& s, float& c, float p)
@@ -128,19 +135,20 @@ __global__ __launch_bounds__(256) void sym_vjp_kernel(const float4* __restrict__
                     const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
                     const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
                     const float y = __builtin_amdgcn_rsqf(d2);
-                    const float y2 = y * y, y3 = y2 * y, y5m3 = (-3.f * y2) * y3;
-                    const float wx = w_fast(mi[k], ax, sm, bx[k]);
-                    const float wy = w_fast(mi[k], ay, sm, by[k]);
-                    const float wz = w_fast(mi[k], az, sm, bz[k]);
-                    const float q = y5m3 * __builtin_fmaf(rx, wx, __builtin_fmaf(ry, wy, rz * wz));
-                    p[k][0] = __builtin_fmaf(q, rx, __builtin_fmaf(y3, wx, p[k][0]));
-                    p[k][1] = __builtin_fmaf(q, ry, __builtin_fmaf(y3, wy, p[k][1]));
-                    p[k][2] = __builtin_fmaf(q, rz, __builtin_fmaf(y3, wz, p[k][2]));
-                    hx = __builtin_fmaf(-q, rx, __builtin_fmaf(-y3, wx, hx));
-                    hy = __builtin_fmaf(-q, ry, __builtin_fmaf(-y3, wy, hy));
-                    hz = __builtin_fmaf(-q, rz, __builtin_fmaf(-y3, wz, hz));
+                    const float y2 = y * y, y3 = y2 * y;
+                    // J w = y^3 (w - 3 y^2 (r.w) r), r.w = m_i (a_j.r) - m_j (a_i.r)
                     const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
                     const float ari = __builtin_fmaf(bx[k], rx, __builtin_fmaf(by[k], ry, bz[k] * rz));
+                    const float s3 = (-3.f * y2) * __builtin_fmaf(mi[k], arj, -(sm * ari));
+                    const float tx = __builtin_fmaf(s3, rx, w_fast(mi[k], ax, sm, bx[k]));
+                    const float ty = __builtin_fmaf(s3, ry, w_fast(mi[k], ay, sm, by[k]));
+                    const float tz = __builtin_fmaf(s3, rz, w_fast(mi[k], az, sm, bz[k]));
+                    p[k][0] = __builtin_fmaf(y3, tx, p[k][0]);
+                    p[k][1] = __builtin_fmaf(y3, ty, p[k][1]);
+                    p[k][2] = __builtin_fmaf(y3, tz, p[k][2]);
+                    hx = __builtin_fmaf(-y3, tx, hx);
+                    hy = __builtin_fmaf(-y3, ty, hy);
+                    hz = __builtin_fmaf(-y3, tz, hz);
                     p[k][3] = __builtin_fmaf(-y3, arj, p[k][3]);
                     hm = __builtin_fmaf(y3, ari, hm);
                 }
