@@ -162,8 +162,8 @@ typedef struct utcnb_batch utcnb_batch;
 
 /*
  * Many independent systems of uneven size in one device allocation.
- * sizes[nsys]; every system shares cfg's precision (MIXED or FP64), kick,
- * eps2, damping and dt (cfg->n is ignored).  The reference's analog is
+ * sizes[nsys]; every system shares cfg's precision (MIXED only), kick, eps2,
+ * damping and dt (cfg->n is ignored).  The reference's analog is
  * benchapps/nbody/utc/nbody_mtask.cc:185-231 (one task and stream per system).
  */
 int utcnb_batch_create(int nsys, const int32_t* sizes, const utcnb_config* cfg, void* stream,
@@ -178,8 +178,12 @@ int utcnb_batch_get_state(utcnb_batch* b, double* pos4, double* vel4);
  *         replayed; every run starts from the FP64 state);
  * mode 1: per-system launches of the single-system kernels (the baseline);
  * mode 2: the persistent kernels of mode 0 launched eagerly (no graph).
- * cfg->tile of the batch: 0 = pick the waves per work item (4, 8, 16) by
- * timing on the first mode 0/2 run, 1..3 = fixed 4, 8 or 16.
+ * cfg->tile of the batch selects the batched kernels: 1..3 = one-pass with 4,
+ * 8 or 16 waves per (system, 64-target tile) work item on persistent
+ * workgroups; 4 = symmetric (each unordered pair once) with (system, wave
+ * block, split) items on persistent waves; 5 = symmetric, one wave per item;
+ * 0 = time all five on the first mode 0/2 run and keep the fastest.  With
+ * tile 0, 4 or 5 every system is padded to a multiple of 256 bodies, else 64.
  * The batched adjoint supports the physical kick only.
  * When target3 is non-NULL the adjoint of the same loss as utcnb_adjoint runs
  * after the forward steps (checkpoint every ckpt_every) and losses[nsys],
@@ -189,6 +193,10 @@ int utcnb_batch_get_state(utcnb_batch* b, double* pos4, double* vel4);
 int utcnb_batch_run(utcnb_batch* b, int nsteps, int mode, int ckpt_every,
                     const double* target3, double* losses, double* dx0, double* dv0,
                     double* dm, double* ms);
+/* The batch tile (1..5, see above) the batch runs: the configured one, the
+ * tuned one after the first mode 0/2 run, or 0 before tuning.  No reference
+ * counterpart. */
+int utcnb_batch_get_tile(const utcnb_batch* b, int* tile);
 
 /* ---- tile autotuner ----------------------------------------------------- */
 /*

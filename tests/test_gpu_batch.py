@@ -41,8 +41,10 @@ def slices(sizes):
 SIZES = [256, 300, 1000, 64, 1, 2048, 777, 4096, 130, 512]
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
 def test_batched_equals_single_system_batches(tile):
+    """Tiles 1..3: one-pass kernels, 4 and 5: symmetric kernels (each system's
+    partial sums and their reduction order depend on that system alone)."""
     eps2, dt, steps = 1e-4, 1e-3, 4
     pos, vel = make_systems(SIZES)
     say(f"case: batch vs singletons tile={tile}")
@@ -59,11 +61,12 @@ def test_batched_equals_single_system_batches(tile):
         assert np.array_equal(bv[sl], v1), k
 
 
-def test_graph_replays_equal_eager():
+@pytest.mark.parametrize("tile", [2, 4])
+def test_graph_replays_equal_eager(tile):
     eps2, dt, steps = 1e-4, 1e-3, 5
     pos, vel = make_systems(SIZES, 7)
-    say("case: graph replays vs eager")
-    with U.Batch(SIZES, eps2, dt, tile=2) as g, U.Batch(SIZES, eps2, dt, tile=2) as e:
+    say(f"case: graph replays vs eager tile={tile}")
+    with U.Batch(SIZES, eps2, dt, tile=tile) as g, U.Batch(SIZES, eps2, dt, tile=tile) as e:
         g.set_state(pos, vel)
         e.set_state(pos, vel)
         for rep in range(3):
@@ -75,12 +78,13 @@ def test_graph_replays_equal_eager():
             assert np.array_equal(gv, ev), rep
 
 
-def test_batched_vs_per_system_and_oracle():
+@pytest.mark.parametrize("tile", [0, 1, 4, 5])
+def test_batched_vs_per_system_and_oracle(tile):
     eps2, dt, steps = 1e-4, 1e-3, 4
     sizes = [256, 1024, 333, 2048]
     pos, vel = make_systems(sizes, 21)
-    say("case: batch vs per-system vs oracle")
-    with U.Batch(sizes, eps2, dt) as b:
+    say(f"case: batch vs per-system vs oracle tile={tile}")
+    with U.Batch(sizes, eps2, dt, tile=tile) as b:
         b.set_state(pos, vel)
         b.run(steps, mode=0)
         gp, _ = b.get_state()
@@ -93,13 +97,14 @@ def test_batched_vs_per_system_and_oracle():
         assert relnorm(pp[sl, :3], p64[:, :3]) <= 1e-9
 
 
-def test_batched_adjoint_matches_single_system():
+@pytest.mark.parametrize("tile", [0, 2, 4, 5])
+def test_batched_adjoint_matches_single_system(tile):
     eps2, dt, steps, ck = 1e-4, 1e-3, 6, 4
     sizes = [256, 700, 1500, 64]
     pos, vel = make_systems(sizes, 33)
     tgt = U.generate(U.GEN_TARGETS, sum(sizes), 5, 0.5)
-    say("case: batched adjoint")
-    with U.Batch(sizes, eps2, dt) as b:
+    say(f"case: batched adjoint tile={tile}")
+    with U.Batch(sizes, eps2, dt, tile=tile) as b:
         b.set_state(pos, vel)
         ms, losses, dx, dv, dm = b.run(steps, mode=0, ckpt_every=ck, target3=tgt)
         b.set_state(pos, vel)
@@ -121,6 +126,8 @@ def test_batch_rejects_bad_arguments():
         U.Batch([16, 0, 5], 1e-4, 1e-3)
     with pytest.raises(U.UtcnbError):
         U.Batch([16], 0.0, 1e-3)
+    with pytest.raises(U.UtcnbError):
+        U.Batch([16], 1e-4, 1e-3, tile=6)
     with U.Batch([100], 1e-4, 1e-3, kick=U.KICK_REFERENCE) as b:
         p, v = make_systems([100])
         b.set_state(p, v)

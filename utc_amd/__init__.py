@@ -32,7 +32,7 @@ EXPORTS = [
     "utcnb_set_state", "utcnb_get_state", "utcnb_step", "utcnb_sync", "utcnb_accel",
     "utcnb_diagnostics", "utcnb_adjoint", "utcnb_last_kernel_ms", "utcnb_get_tile", "utcnb_batch_create",
     "utcnb_batch_destroy", "utcnb_batch_set_state", "utcnb_batch_get_state",
-    "utcnb_batch_run", "utcnb_autotune", "utcnb_tile_count", "utcnb_tile_name",
+    "utcnb_batch_run", "utcnb_batch_get_tile", "utcnb_autotune", "utcnb_tile_count", "utcnb_tile_name",
     "utcnb_rng_seed", "utcnb_rng_next", "utcnb_splitmix64", "utcnb_generate",
     "utcnb_last_error", "utcnb_version", "utcnb_device_info",
 ]
@@ -83,6 +83,7 @@ def _proto(lib):
         "utcnb_batch_set_state": (C.c_int, [_vp, _vp, _vp]),
         "utcnb_batch_get_state": (C.c_int, [_vp, _vp, _vp]),
         "utcnb_batch_run": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _dp]),
+        "utcnb_batch_get_tile": (C.c_int, [_vp, P(C.c_int)]),
         "utcnb_autotune": (C.c_int, [C.c_int, C.c_int, C.c_char_p, P(C.c_int), _dp]),
         "utcnb_tile_count": (C.c_int, []),
         "utcnb_tile_name": (C.c_char_p, [C.c_int]),
@@ -270,6 +271,18 @@ class Batch:
         v = np.zeros((self.total, 4))
         check(lib().utcnb_batch_get_state(self.h, ptr(p), ptr(v)), "utcnb_batch_get_state")
         return p, v
+
+    BATCH_TILES = {0: "not tuned yet", 1: "one-pass, 4 waves per item", 2: "one-pass, 8 waves per item",
+                   3: "one-pass, 16 waves per item", 4: "symmetric, persistent waves",
+                   5: "symmetric, one wave per item"}
+
+    def tile(self):
+        t = C.c_int()
+        check(lib().utcnb_batch_get_tile(self.h, C.byref(t)), "utcnb_batch_get_tile")
+        return t.value
+
+    def kernel_name(self):
+        return self.BATCH_TILES[self.tile()]
 
     def run(self, nsteps, mode=0, ckpt_every=0, target3=None):
         """Returns ms, or (ms, losses, dx, dv, dm) when target3 is given."""

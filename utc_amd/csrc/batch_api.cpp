@@ -23,7 +23,13 @@ using namespace utcnb;
 
 namespace {
 
-constexpr int kWs[3] = {4, 8, 16};  // waves per work item (batch tiles 1..3)
+constexpr int kWs[3] = {4, 8, 16};  // waves per work item of the one-pass kernels (batch tiles 1..3)
+// Batch tiles 4 and 5: the symmetric kernels (sym.hip, sym_vjp.hip), work items
+// (system, wave block, split) on persistent waves in a longest-first static
+// schedule (4), or one wave per item, longest first, for the hardware
+// dispatcher to place (5).
+constexpr int kSymPersist = 3, kSymFlat = 4, kSels = 5;
+constexpr int kSymUnitsPerItem = 2;  // work units (K source blocks of 64) per symmetric item
 
 template <typename T>
 int dalloc(T** p, size_t count)
@@ -65,7 +71,24 @@ struct utcnb_batch {
     int2* d_items[3] = {nullptr, nullptr, nullptr};
     int* d_start[3] = {nullptr, nullptr, nullptr};
     int nblocks[3] = {0, 0, 0};
-    int wsel = -1;  // index into kWs, -1 = not tuned yet
+    int wsel = -1;  // kernel selection 0..kSels-1 (batch tile - 1), -1 = not tuned yet
+    int align = kChunk;  // systems start on multiples of this (kBatchSymAlign for the symmetric kernels)
+    // symmetric kernels: per-system geometry and partial sums
+    int* d_nw = nullptr;
+    int* d_rs = nullptr;
+    long long* d_ipo = nullptr;
+    long long* d_jpo = nullptr;
+    long long* d_vipo = nullptr;
+    long long* d_vjpo = nullptr;
+    int* d_sysblk = nullptr;
+    int4* d_sitems[2] = {nullptr, nullptr};
+    int* d_sstart[2] = {nullptr, nullptr};
+    int snwaves[2] = {0, 0};
+    float* sipart = nullptr;
+    float* sjpart = nullptr;
+    size_t svi_count = 0, svj_count = 0;  // VJP partials (float4), allocated with the adjoint workspace
+    float4* svipart = nullptr;
+    float4* svjpart = nullptr;
     bool has_state = false;
     // adjoint workspace
     int nck = 0, seg = 0;
@@ -86,6 +109,12 @@ struct utcnb_batch {
     {
         return BatchGeom{d_off, d_n, d_chunks, d_items[wi], d_start[wi]};
     }
+    BatchSymGeom sym_geom(int k) const
+    {
+        return BatchSymGeom{d_sitems[k], d_sstart[k], d_off, d_n, d_nw, d_rs, d_ipo, d_jpo, d_vipo, d_vjpo,
+                            d_sysblk};
+    }
+    bool has_sym() const { return align == kBatchSymAlign; }
 };
 
 namespace {
@@ -104,6 +133,8 @@ void release_adj(utcnb_batch* b)
     dfree(b->mbar);
     dfree(b->tgt);
     dfree(b->loss);
+    dfree(b->svipart);
+    dfree(b->svjpart);
     b->nck = b->seg = 0;
 }
 
@@ -124,6 +155,19 @@ void release(utcnb_batch* b)
         dfree(b->d_items[w]);
         dfree(b->d_start[w]);
     }
+    dfree(b->d_nw);
+    dfree(b->d_rs);
+    dfree(b->d_ipo);
+    dfree(b->d_jpo);
+    dfree(b->d_vipo);
+    dfree(b->d_vjpo);
+    dfree(b->d_sysblk);
+    for (int k = 0; k < 2; ++k) {
+        dfree(b->d_sitems[k]);
+        dfree(b->d_sstart[k]);
+    }
+    dfree(b->sipart);
+    dfree(b->sjpart);
     if (b->ev0)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
@@ -169,6 +213,109 @@ void schedule(const utcnb_batch* b, int W, int blocks, std::vector<int2>& items,
     }
 }
 
+// Symmetric kernels: per system NW = npad / kBatchSymAlign wave blocks, each
+// with NW/2 + 1 work units split over R waves (about kSymUnitsPerItem units
+// each).  Items (system, wave block, split, R) are scheduled longest first,
+// either onto `waves` persistent waves by least load, or one wave each.
+struct SymPlan {
+    std::vector<int> nw, rs, sysblk;
+    std::vector<long long> ipo, jpo, vipo, vjpo;
+    long long ifloats = 0, jfloats = 0, vi4 = 0, vj4 = 0;
+    std::vector<int4> items[2];
+    std::vector<int> start[2];
+};
+
+void plan_sym(const utcnb_batch* b, int persistent_waves, SymPlan& P)
+{
+    struct It {
+        int cost;
+        int4 item;
+    };
+    std::vector<It> all;
+    for (int s = 0; s < b->nsys; ++s) {
+        const int NW = b->npad[s] / kBatchSymAlign;
+        const int units = NW / 2 + 1;
+        const int R = std::max(1, (units + kSymUnitsPerItem - 1) / kSymUnitsPerItem);
+        P.nw.push_back(NW);
+        P.rs.push_back(R);
+        P.ipo.push_back(P.ifloats);
+        P.jpo.push_back(P.jfloats);
+        P.vipo.push_back(P.vi4);
+        P.vjpo.push_back(P.vj4);
+        P.ifloats += (long long)R * b->npad[s] * 3;
+        P.jfloats += (long long)NW * (NW / 2) * kBatchSymAlign * 3;
+        P.vi4 += (long long)R * b->npad[s];
+        P.vj4 += (long long)NW * (NW / 2) * kBatchSymAlign;
+        for (int blk = 0; blk < NW; ++blk)
+            P.sysblk.push_back(s);
+        for (int w = 0; w < NW; ++w)
+            for (int r = 0; r < R; ++r) {
+                const int u0 = (int)((long long)units * r / R), u1 = (int)((long long)units * (r + 1) / R);
+                if (u1 > u0)
+                    all.push_back({u1 - u0, make_int4(s, w, r, R)});
+            }
+    }
+    std::stable_sort(all.begin(), all.end(), [](const It& a, const It& c) { return a.cost > c.cost; });
+    // one wave per item
+    P.items[1].clear();
+    P.start[1].assign(1, 0);
+    for (const It& it : all) {
+        P.items[1].push_back(it.item);
+        P.start[1].push_back((int)P.items[1].size());
+    }
+    // persistent waves, least-loaded first
+    const int W = (int)std::max<size_t>(1, std::min<size_t>(all.size(), (size_t)persistent_waves));
+    using Load = std::pair<long long, int>;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (int k = 0; k < W; ++k)
+        heap.push({0, k});
+    std::vector<std::vector<int4>> per(W);
+    for (const It& it : all) {
+        Load l = heap.top();
+        heap.pop();
+        per[l.second].push_back(it.item);
+        heap.push({l.first + it.cost, l.second});
+    }
+    P.items[0].clear();
+    P.start[0].assign(1, 0);
+    for (int k = 0; k < W; ++k) {
+        P.items[0].insert(P.items[0].end(), per[k].begin(), per[k].end());
+        P.start[0].push_back((int)P.items[0].size());
+    }
+}
+
+template <typename T>
+hipError_t upload(T** d, const std::vector<T>& h)
+{
+    if (dalloc(d, h.size()))
+        return hipErrorOutOfMemory;
+    return hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+}
+
+hipError_t setup_sym(utcnb_batch* b)
+{
+    SymPlan P;
+    plan_sym(b, b->cu_count * 16, P);
+    hipError_t e = upload(&b->d_nw, P.nw);
+    if (e == hipSuccess) e = upload(&b->d_rs, P.rs);
+    if (e == hipSuccess) e = upload(&b->d_ipo, P.ipo);
+    if (e == hipSuccess) e = upload(&b->d_jpo, P.jpo);
+    if (e == hipSuccess) e = upload(&b->d_vipo, P.vipo);
+    if (e == hipSuccess) e = upload(&b->d_vjpo, P.vjpo);
+    if (e == hipSuccess) e = upload(&b->d_sysblk, P.sysblk);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = upload(&b->d_sitems[k], P.items[k]);
+        if (e == hipSuccess)
+            e = upload(&b->d_sstart[k], P.start[k]);
+        b->snwaves[k] = (int)P.start[k].size() - 1;
+    }
+    if (e == hipSuccess && (dalloc(&b->sipart, (size_t)P.ifloats) || dalloc(&b->sjpart, (size_t)P.jfloats)))
+        e = hipErrorOutOfMemory;
+    b->svi_count = (size_t)P.vi4;
+    b->svj_count = (size_t)P.vj4;
+    return e;
+}
+
 int adj_workspace(utcnb_batch* b, int nck, int seg)
 {
     if (b->xbar && b->nck >= nck && b->seg >= seg)
@@ -180,6 +327,8 @@ int adj_workspace(utcnb_batch* b, int nck, int seg)
              dalloc(&b->wsnap[1], T) || dalloc(&b->jrec, 2 * T) || dalloc(&b->xbar, 3 * T) ||
              dalloc(&b->vbar, 3 * T) || dalloc(&b->mbar, T) || dalloc(&b->tgt, 3 * T) ||
              dalloc(&b->loss, (size_t)b->nsys);
+    if (!rc && b->has_sym())
+        rc = dalloc(&b->svipart, b->svi_count) || dalloc(&b->svjpart, b->svj_count);
     if (rc) {
         release_adj(b);
         return fail(UTCNB_ENOMEM, "utcnb_batch_run: adjoint workspace allocation failed");
@@ -208,7 +357,8 @@ int per_system_tile(int npad)
 }
 
 // Enqueue one run (forward nsteps, optionally the adjoint) on b->stream.
-// mode 0: persistent batched kernels with W = kWs[wi]; mode 1: per-system launches.
+// mode 0: batched kernels (one-pass with W = kWs[wi], or symmetric for
+// wi >= kSymPersist); mode 1: per-system launches.
 hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode, int wi)
 {
     hipStream_t st = b->stream;
@@ -224,7 +374,11 @@ hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode,
             return e;              \
     } while (0)
 
+    const int sk = wi - kSymPersist;  // symmetric schedule (0 persistent, 1 one wave per item)
     auto step = [&](const float4* so, float4* sn, double4* p, double4* v) -> hipError_t {
+        if (mode == 0 && wi >= kSymPersist)
+            return launch_batch_sym_step(b->sym_geom(sk), b->snwaves[sk], (int)T, so, sn, p, v, eps2, dt,
+                                         damp, kick, b->sipart, b->sjpart, st);
         if (mode == 0)
             return launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], so, sn, p, v, eps2, dt,
                                      damp, kick, st);
@@ -247,6 +401,9 @@ hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode,
         return hipSuccess;
     };
     auto vjp = [&]() -> hipError_t {
+        if (mode == 0 && wi >= kSymPersist)
+            return launch_batch_sym_vjp(b->sym_geom(sk), b->snwaves[sk], (int)T, b->jrec, eps2, b->svipart,
+                                        b->svjpart, b->xbar, b->mbar, st);
         if (mode == 0)
             return launch_batch_vjp(kWs[wi], b->geom(wi), b->nblocks[wi], b->jrec, eps2, b->xbar,
                                     b->mbar, st);
@@ -318,8 +475,8 @@ int utcnb_batch_create(int nsys, const int32_t* sizes, const utcnb_config* cfg, 
         return fail(UTCNB_EINVAL, "utcnb_batch_create: unknown kick mode");
     if (!(cfg->eps2 > 0.0))
         return fail(UTCNB_EINVAL, "utcnb_batch_create: eps2 must be > 0");
-    if (cfg->tile < 0 || cfg->tile > 3)
-        return fail(UTCNB_EINVAL, "utcnb_batch_create: tile must be 0 (tune) or 1..3");
+    if (cfg->tile < 0 || cfg->tile > kSels)
+        return fail(UTCNB_EINVAL, "utcnb_batch_create: tile must be 0 (tune) or 1..5");
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
@@ -332,6 +489,8 @@ int utcnb_batch_create(int nsys, const int32_t* sizes, const utcnb_config* cfg, 
     b->cfg = *cfg;
     b->cu_count = prop.multiProcessorCount;
     b->nsys = nsys;
+    // the symmetric kernels (candidates when tuning) need systems in whole wave blocks
+    b->align = (cfg->tile == 0 || cfg->tile > kSymPersist) ? kBatchSymAlign : kChunk;
     long long o = 0;
     for (int s = 0; s < nsys; ++s) {
         if (sizes[s] < 1 || sizes[s] > (1 << 24)) {
@@ -339,7 +498,7 @@ int utcnb_batch_create(int nsys, const int32_t* sizes, const utcnb_config* cfg, 
             return fail(UTCNB_EINVAL, "utcnb_batch_create: system sizes must be in [1, 2^24]");
         }
         b->n.push_back(sizes[s]);
-        b->npad.push_back((sizes[s] + kChunk - 1) / kChunk * kChunk);
+        b->npad.push_back((sizes[s] + b->align - 1) / b->align * b->align);
         b->off.push_back((int)o);
         o += b->npad.back();
         b->total_real += sizes[s];
@@ -389,6 +548,8 @@ int utcnb_batch_create(int nsys, const int32_t* sizes, const utcnb_config* cfg, 
             if (e == hipSuccess)
                 e = hipMemcpy(b->d_start[wi], start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice);
         }
+        if (e == hipSuccess && b->has_sym())
+            e = setup_sym(b);
         if (e == hipSuccess)
             e = launch_batch_pad(b->geom(0), nsys, b->pos, b->vel, b->snap[0], b->snap[1], b->stream);
         if (e == hipSuccess && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess))
@@ -473,7 +634,9 @@ int utcnb_batch_get_state(utcnb_batch* b, double* pos4, double* vel4)
     return UTCNB_OK;
 }
 
-// Pick W by timing one forward step of every tile on a copy of the state.
+// Pick the batched kernel (one-pass W = 4, 8, 16; symmetric persistent or one
+// wave per item when the layout allows) by timing forward steps of each on a
+// copy of the state.
 static int tune_w(utcnb_batch* b)
 {
     const size_t T = (size_t)b->total;
@@ -486,15 +649,20 @@ static int tune_w(utcnb_batch* b)
     UTCNB_HIP(hipMemcpyAsync(sv, b->vel, T * sizeof(double4), hipMemcpyDeviceToDevice, b->stream));
     float best = 1e30f;
     int bw = 1;
-    for (int wi = 0; wi < 3; ++wi) {
-        UTCNB_HIP(launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], b->snap[0], b->snap[1], b->pos,
-                                    b->vel, (float)b->cfg.eps2, b->cfg.dt, b->cfg.damping, b->cfg.kick,
-                                    b->stream));
+    const float eps2 = (float)b->cfg.eps2;
+    auto one = [&](int wi) -> hipError_t {
+        if (wi >= kSymPersist)
+            return launch_batch_sym_step(b->sym_geom(wi - kSymPersist), b->snwaves[wi - kSymPersist],
+                                         (int)T, b->snap[0], b->snap[1], b->pos, b->vel, eps2, b->cfg.dt,
+                                         b->cfg.damping, b->cfg.kick, b->sipart, b->sjpart, b->stream);
+        return launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], b->snap[0], b->snap[1], b->pos,
+                                 b->vel, eps2, b->cfg.dt, b->cfg.damping, b->cfg.kick, b->stream);
+    };
+    for (int wi = 0; wi < (b->has_sym() ? kSels : 3); ++wi) {
+        UTCNB_HIP(one(wi));
         UTCNB_HIP(hipEventRecord(b->ev0, b->stream));
         for (int r = 0; r < 3; ++r)
-            UTCNB_HIP(launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], b->snap[0], b->snap[1],
-                                        b->pos, b->vel, (float)b->cfg.eps2, b->cfg.dt, b->cfg.damping,
-                                        b->cfg.kick, b->stream));
+            UTCNB_HIP(one(wi));
         UTCNB_HIP(hipEventRecord(b->ev1, b->stream));
         UTCNB_HIP(hipEventSynchronize(b->ev1));
         float ms = 0.f;
@@ -510,6 +678,14 @@ static int tune_w(utcnb_batch* b)
     dfree(sp);
     dfree(sv);
     b->wsel = bw;
+    return UTCNB_OK;
+}
+
+int utcnb_batch_get_tile(const utcnb_batch* b, int* tile)
+{
+    if (!b || !tile)
+        return fail(UTCNB_EINVAL, "utcnb_batch_get_tile: null argument");
+    *tile = b->wsel + 1;
     return UTCNB_OK;
 }
 

@@ -125,6 +125,30 @@ hipError_t launch_batch_step(int W, const BatchGeom& g, int nblocks, const float
                              double damping, int kick, hipStream_t st);
 hipError_t launch_batch_vjp(int W, const BatchGeom& g, int nblocks, const float4* jrec, float eps2,
                             double* xbar3, double* mbar, hipStream_t st);
+// Batched symmetric kernels (sym.hip, sym_vjp.hip): K = 4 targets per lane;
+// every system starts on a multiple of kBatchSymAlign bodies and is padded to
+// one, so it is a whole number of wave blocks.
+constexpr int kBatchSymK = 4;
+constexpr int kBatchSymAlign = 64 * kBatchSymK;
+struct BatchSymGeom {
+    const int4* items;        // (system, wave block, split, splits of the system)
+    const int* start;         // items of wave g: [start[g], start[g+1])
+    const int* off;           // first body of each system (padded layout)
+    const int* n;             // bodies per system
+    const int* nw;            // wave blocks per system
+    const int* rs;            // splits per system
+    const long long* ipo;     // force partials: first float of the system's target-side block
+    const long long* jpo;     //   ... and of its reaction block
+    const long long* vipo;    // VJP partials (float4 units), target side
+    const long long* vjpo;    //   ... reactions
+    const int* sys_of_block;  // system of each kBatchSymAlign-body block
+};
+hipError_t launch_batch_sym_step(const BatchSymGeom& g, int nwaves, int total, const float4* snap_old,
+                                 float4* snap_new, double4* pos, double4* vel, float eps2, double dt,
+                                 double damping, int kick, float* ipart, float* jpart, hipStream_t st);
+hipError_t launch_batch_sym_vjp(const BatchSymGeom& g, int nwaves, int total, const float4* jrec,
+                                float eps2, float4* ipart, float4* jpart, double* xbar3, double* mbar,
+                                hipStream_t st);
 hipError_t launch_batch_loss(const BatchGeom& g, int nsys, const double4* pos, const double* tgt3,
                              double* xbar3, double* vbar3, double* mbar, double* loss,
                              hipStream_t st);

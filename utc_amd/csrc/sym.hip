@@ -35,17 +35,14 @@ __device__ __forceinline__ float wave_ror1(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x13C /* wave_ror:1 */, 0xF, 0xF, false));
 }
 
-// ipart: [R][n_pad][3] target-side sums; jpart: [NW][NW/2][64K][3] reactions.
+// The work of wave r of wave block w of one system: src = the system's
+// padded bodies (NW wave blocks of 64K); ipart: [R][n_pad][3] target-side
+// sums; jpart: [NW][NW/2][64K][3] reactions.
 template <int K, bool NEWTON>
-__global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict__ src, int NW, int R,
-                                                        float eps2, float* __restrict__ ipart,
-                                                        float* __restrict__ jpart)
+__device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW, int R, int w, int r,
+                                         float eps2, float* __restrict__ ipart,
+                                         float* __restrict__ jpart, int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (gw >= NW * R)
-        return;  // whole waves only
-    const int w = gw / R, r = gw % R;
     const int half = NW / 2;
     const int n_pad = NW * 64 * K;
     float xi[K], yi[K], zi[K], mi[K], ax[K], ay[K], az[K], cx[K], cy[K], cz[K];
@@ -141,15 +138,42 @@ __global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict
     }
 }
 
-// Sum of the partials of body b in FP64, in a fixed order, then the epilogue.
-template <class Epi>
-__global__ __launch_bounds__(256) void sym_reduce_kernel(const float* __restrict__ ipart,
-                                                         const float* __restrict__ jpart, int n, int NW,
-                                                         int R, int K, Epi epi)
+// One system: wave gw = w * R + r.
+template <int K, bool NEWTON>
+__global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict__ src, int NW, int R,
+                                                        float eps2, float* __restrict__ ipart,
+                                                        float* __restrict__ jpart)
 {
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= n)
-        return;
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (gw >= NW * R)
+        return;  // whole waves only
+    sym_wave<K, NEWTON>(src, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
+}
+
+// Batched systems: wave gw runs the items [start[gw], start[gw+1]), each
+// (system, wave block, split) of a system laid out as for one system.
+template <int K>
+__global__ __launch_bounds__(256) void batch_sym_kernel(BatchSymGeom g, int nwaves, const float4* __restrict__ snap,
+                                                        float eps2, float* __restrict__ ipart,
+                                                        float* __restrict__ jpart)
+{
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (gw >= nwaves)
+        return;  // whole waves only
+    const int lane = threadIdx.x & 63;
+    for (int it = g.start[gw]; it < g.start[gw + 1]; ++it) {
+        const int4 item = g.items[it];  // (system, wave block, split, splits)
+        const int s = item.x;
+        sym_wave<K, false>(snap + g.off[s], g.nw[s], item.w, item.y, item.z, eps2, ipart + g.ipo[s],
+                           jpart + g.jpo[s], lane);
+    }
+}
+
+// Sum of the partials of body b (index within its system) in FP64, in a
+// fixed order: the R target-side partials, then the reactions by distance d.
+__device__ __forceinline__ void sym_sum(const float* __restrict__ ipart, const float* __restrict__ jpart,
+                                        int b, int NW, int R, int K, double s[3])
+{
     const int n_pad = NW * 64 * K;
     const int half = NW / 2, v = b / (64 * K), loc = b - v * (64 * K);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
@@ -170,9 +194,48 @@ __global__ __launch_bounds__(256) void sym_reduce_kernel(const float* __restrict
         s1 += p[1];
         s2 += p[2];
     }
-    // the epilogues take (hi, c) with value hi - c; split the FP64 sum exactly
-    const float h0 = (float)s0, h1 = (float)s1, h2 = (float)s2;
-    epi(b, h0, h1, h2, (float)((double)h0 - s0), (float)((double)h1 - s1), (float)((double)h2 - s2));
+    s[0] = s0;
+    s[1] = s1;
+    s[2] = s2;
+}
+
+// The epilogues take (hi, c) with value hi - c; split an FP64 sum exactly.
+template <class Epi>
+__device__ __forceinline__ void sym_epilogue(const Epi& epi, int i, const double s[3])
+{
+    const float h0 = (float)s[0], h1 = (float)s[1], h2 = (float)s[2];
+    epi(i, h0, h1, h2, (float)((double)h0 - s[0]), (float)((double)h1 - s[1]), (float)((double)h2 - s[2]));
+}
+
+template <class Epi>
+__global__ __launch_bounds__(256) void sym_reduce_kernel(const float* __restrict__ ipart,
+                                                         const float* __restrict__ jpart, int n, int NW,
+                                                         int R, int K, Epi epi)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= n)
+        return;
+    double s[3];
+    sym_sum(ipart, jpart, b, NW, R, K, s);
+    sym_epilogue(epi, b, s);
+}
+
+// Batched systems: body b of the concatenated padded layout (systems start on
+// multiples of kBatchSymAlign, so block b / kBatchSymAlign names the system).
+__global__ __launch_bounds__(256) void batch_sym_reduce_kernel(BatchSymGeom g, int total, int K,
+                                                               const float* __restrict__ ipart,
+                                                               const float* __restrict__ jpart, EpiMixed epi)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= total)
+        return;
+    const int s = g.sys_of_block[b / kBatchSymAlign];
+    const int loc = b - g.off[s];
+    if (loc >= g.n[s])
+        return;  // padding bodies keep their far-away zero-mass records
+    double sum[3];
+    sym_sum(ipart + g.ipo[s], jpart + g.jpo[s], loc, g.nw[s], g.rs[s], K, sum);
+    sym_epilogue(epi, b, sum);
 }
 
 size_t sym_ipart_floats(int n_pad, int K, int R) { (void)K; return (size_t)R * n_pad * 3; }
@@ -210,6 +273,21 @@ hipError_t launch_sym(const Tile& t, bool newton, const float4* src, int n, int 
         return e;
     hipLaunchKernelGGL((sym_reduce_kernel<Epi>), dim3((n + 255) / 256), dim3(256), 0, st, ipart, jpart, n,
                        NW, R, K, epi);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_sym_step(const BatchSymGeom& g, int nwaves, int total, const float4* snap_old,
+                                 float4* snap_new, double4* pos, double4* vel, float eps2, double dt,
+                                 double damping, int kick, float* ipart, float* jpart, hipStream_t st)
+{
+    hipLaunchKernelGGL((batch_sym_kernel<kBatchSymK>), dim3((nwaves + 3) / 4), dim3(256), 0, st, g, nwaves,
+                       snap_old, eps2, ipart, jpart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    EpiMixed epi{pos, vel, snap_new, dt, damping, kick};
+    hipLaunchKernelGGL(batch_sym_reduce_kernel, dim3((total + 255) / 256), dim3(256), 0, st, g, total,
+                       kBatchSymK, ipart, jpart, epi);
     return hipGetLastError();
 }
 

@@ -46,18 +46,14 @@ __device__ __forceinline__ void kahan1(float& s, float& c, float p)
 
 }  // namespace
 
-// jrec: two float4 per body {x,y,z,m} {ax,ay,az,0}, n_pad = NW*64*K bodies.
-// ipart: [R][n_pad] float4; jpart: [NW][NW/2][64K] float4.
+// The work of wave r of wave block w of one system.  jrec: two float4 per
+// body {x,y,z,m} {ax,ay,az,0}, n_pad = NW*64*K bodies; ipart: [R][n_pad]
+// float4; jpart: [NW][NW/2][64K] float4.
 template <int K>
-__global__ __launch_bounds__(256) void sym_vjp_kernel(const float4* __restrict__ jrec, int NW, int R,
-                                                      float eps2, float4* __restrict__ ipart,
-                                                      float4* __restrict__ jpart)
+__device__ __forceinline__ void sym_vjp_wave(const float4* __restrict__ jrec, int NW, int R, int w, int r,
+                                             float eps2, float4* __restrict__ ipart,
+                                             float4* __restrict__ jpart, int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (gw >= NW * R)
-        return;  // whole waves only
-    const int w = gw / R, r = gw % R;
     const int half = NW / 2;
     const int n_pad = NW * 64 * K;
     float xi[K], yi[K], zi[K], mi[K], bx[K], by[K], bz[K];
@@ -170,16 +166,41 @@ __global__ __launch_bounds__(256) void sym_vjp_kernel(const float4* __restrict__
             make_float4(g[k][0] - c[k][0], g[k][1] - c[k][1], g[k][2] - c[k][2], g[k][3] - c[k][3]);
 }
 
-// xbar_b += sum of body b's partials, mbar_b likewise; FP64, fixed order.
-__global__ __launch_bounds__(256) void sym_vjp_reduce_kernel(const float4* __restrict__ ipart,
-                                                             const float4* __restrict__ jpart, int n,
-                                                             int NW, int R, int K,
-                                                             double* __restrict__ xbar3,
-                                                             double* __restrict__ mbar)
+// One system: wave gw = w * R + r.
+template <int K>
+__global__ __launch_bounds__(256) void sym_vjp_kernel(const float4* __restrict__ jrec, int NW, int R,
+                                                      float eps2, float4* __restrict__ ipart,
+                                                      float4* __restrict__ jpart)
 {
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= n)
-        return;
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (gw >= NW * R)
+        return;  // whole waves only
+    sym_vjp_wave<K>(jrec, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
+}
+
+// Batched systems: wave gw runs the items [start[gw], start[gw+1]).
+template <int K>
+__global__ __launch_bounds__(256) void batch_sym_vjp_kernel(BatchSymGeom g, int nwaves,
+                                                            const float4* __restrict__ jrec, float eps2,
+                                                            float4* __restrict__ ipart,
+                                                            float4* __restrict__ jpart)
+{
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (gw >= nwaves)
+        return;  // whole waves only
+    const int lane = threadIdx.x & 63;
+    for (int it = g.start[gw]; it < g.start[gw + 1]; ++it) {
+        const int4 item = g.items[it];  // (system, wave block, split, splits)
+        const int s = item.x;
+        sym_vjp_wave<K>(jrec + 2 * (size_t)g.off[s], g.nw[s], item.w, item.y, item.z, eps2,
+                        ipart + g.vipo[s], jpart + g.vjpo[s], lane);
+    }
+}
+
+// Sum of body b's partials (index within its system), FP64, fixed order.
+__device__ __forceinline__ void sym_vjp_sum(const float4* __restrict__ ipart, const float4* __restrict__ jpart,
+                                            int b, int NW, int R, int K, double s[4])
+{
     const int n_pad = NW * 64 * K;
     const int half = NW / 2, v = b / (64 * K), loc = b - v * (64 * K);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -196,10 +217,50 @@ __global__ __launch_bounds__(256) void sym_vjp_reduce_kernel(const float4* __res
         const float4 p = jpart[((size_t)w * half + d) * (64 * K) + loc];
         s0 += p.x; s1 += p.y; s2 += p.z; s3 += p.w;
     }
-    xbar3[3 * (size_t)b + 0] += s0;
-    xbar3[3 * (size_t)b + 1] += s1;
-    xbar3[3 * (size_t)b + 2] += s2;
-    mbar[b] += s3;
+    s[0] = s0;
+    s[1] = s1;
+    s[2] = s2;
+    s[3] = s3;
+}
+
+// xbar_b += sum of body b's partials, mbar_b likewise.
+__global__ __launch_bounds__(256) void sym_vjp_reduce_kernel(const float4* __restrict__ ipart,
+                                                             const float4* __restrict__ jpart, int n,
+                                                             int NW, int R, int K,
+                                                             double* __restrict__ xbar3,
+                                                             double* __restrict__ mbar)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= n)
+        return;
+    double s[4];
+    sym_vjp_sum(ipart, jpart, b, NW, R, K, s);
+    xbar3[3 * (size_t)b + 0] += s[0];
+    xbar3[3 * (size_t)b + 1] += s[1];
+    xbar3[3 * (size_t)b + 2] += s[2];
+    mbar[b] += s[3];
+}
+
+// Batched systems: body b of the concatenated padded layout.
+__global__ __launch_bounds__(256) void batch_sym_vjp_reduce_kernel(BatchSymGeom g, int total, int K,
+                                                                   const float4* __restrict__ ipart,
+                                                                   const float4* __restrict__ jpart,
+                                                                   double* __restrict__ xbar3,
+                                                                   double* __restrict__ mbar)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= total)
+        return;
+    const int s = g.sys_of_block[b / kBatchSymAlign];
+    const int loc = b - g.off[s];
+    if (loc >= g.n[s])
+        return;
+    double sum[4];
+    sym_vjp_sum(ipart + g.vipo[s], jpart + g.vjpo[s], loc, g.nw[s], g.rs[s], K, sum);
+    xbar3[3 * (size_t)b + 0] += sum[0];
+    xbar3[3 * (size_t)b + 1] += sum[1];
+    xbar3[3 * (size_t)b + 2] += sum[2];
+    mbar[b] += sum[3];
 }
 
 constexpr int kSymVjpK = 4;
@@ -240,6 +301,21 @@ hipError_t launch_sym_vjp(const float4* jrec, int n, int n_pad, float eps2, int 
         return e;
     hipLaunchKernelGGL(sym_vjp_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ipart, jpart, n,
                        NW, R, kSymVjpK, xbar3, mbar);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_sym_vjp(const BatchSymGeom& g, int nwaves, int total, const float4* jrec,
+                                float eps2, float4* ipart, float4* jpart, double* xbar3, double* mbar,
+                                hipStream_t st)
+{
+    static_assert(kBatchSymK == kSymVjpK, "batched and single-system symmetric VJP share K");
+    hipLaunchKernelGGL((batch_sym_vjp_kernel<kBatchSymK>), dim3((nwaves + 3) / 4), dim3(256), 0, st, g,
+                       nwaves, jrec, eps2, ipart, jpart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(batch_sym_vjp_reduce_kernel, dim3((total + 255) / 256), dim3(256), 0, st, g, total,
+                       kBatchSymK, ipart, jpart, xbar3, mbar);
     return hipGetLastError();
 }
 
