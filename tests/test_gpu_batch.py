@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import utc_amd as U
-from nbcases import relnorm
+from nbcases import config5_sizes, relnorm
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -119,6 +119,36 @@ def test_batched_adjoint_matches_single_system(tile):
             assert relnorm(mine[0][sl], gx) <= 1e-5
             assert relnorm(mine[1][sl], gv) <= 1e-5
             assert relnorm(mine[2][sl], gm) <= 1e-5
+
+
+def test_config5_full_batch_graph_vs_per_system():
+    """Config 5 at full size: 256 Plummer systems (sizes log-uniform in
+    [256, 8192]), 20 steps forward and forward + adjoint.  The tuned graph
+    of batched kernels against per-system launches of the single-system
+    kernels: positions within 1e-9 (both FP64 state, FP32 pairs), losses
+    within 1e-9 and gradients within 1e-5 relative, per system."""
+    sizes = config5_sizes(256, 0)
+    pos, vel = make_systems(sizes, 1000)
+    tgt = U.generate(U.GEN_TARGETS, sum(sizes), 77, 0.5)
+    say(f"case: config5 full batch, {len(sizes)} systems, {sum(sizes)} bodies")
+    with U.Batch(sizes, 1e-4, 1e-3, tile=0) as bg, U.Batch(sizes, 1e-4, 1e-3, tile=1) as bp:
+        bg.set_state(pos, vel)
+        bg.run(20, mode=0)
+        gp, _ = bg.get_state()
+        say(f"  graph kernel: {bg.kernel_name()}")
+        bp.set_state(pos, vel)
+        bp.run(20, mode=1)
+        pp, _ = bp.get_state()
+        bg.set_state(pos, vel)
+        _, lg, gx, gv, gm = bg.run(20, mode=0, ckpt_every=10, target3=tgt)
+        bp.set_state(pos, vel)
+        _, lp, px, pv, pm = bp.run(20, mode=1, ckpt_every=10, target3=tgt)
+    for k, sl in enumerate(slices(sizes)):
+        assert relnorm(gp[sl, :3], pp[sl, :3]) <= 1e-9, k
+        assert lg[k] == pytest.approx(lp[k], rel=1e-9), k
+        assert relnorm(gx[sl], px[sl]) <= 1e-5, k
+        assert relnorm(gv[sl], pv[sl]) <= 1e-5, k
+        assert relnorm(gm[sl], pm[sl]) <= 1e-5, k
 
 
 def test_batch_rejects_bad_arguments():

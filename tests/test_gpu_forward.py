@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import utc_amd as U
-from nbcases import config1, plummer, relnorm
+from nbcases import config1, config2, config3, plummer, relnorm
 from oracle import oracle as O
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -201,3 +201,37 @@ def test_diagnostics_match_oracle():
         assert abs(k - K) <= 1e-13 * abs(K)
         assert abs(w - W) <= 1e-12 * abs(W)
         assert np.abs(p - P).max() <= 1e-15
+
+
+@pytest.mark.parametrize("which", ["config2", "config3"])
+def test_full_size_accuracy_against_fp64(which):
+    """Configs 2 and 3 at full size (N = 65536 and 262144, Plummer): final
+    positions of the mixed path (default tile, symmetric) and of plain FP32
+    against the on-device FP64 path (itself equal to the CPU FP64 oracle to
+    1e-14 at N = 2048, test_fp64_field_matches_cpu_fp64); target <= 1e-6
+    relative.  The mixed run's total energy and momentum stay put."""
+    c = config2() if which == "config2" else config3()
+    n, eps2, steps = len(c["pos"]), c["eps"] ** 2, c["steps"]
+    say(f"case: {which} full size n={n} steps={steps}")
+    with U.System(n, U.PREC_FP64, eps2, c["dt"]) as s64:
+        s64.set_state(c["pos"], c["vel"])
+        s64.step(steps)
+        p64, _ = s64.get_state()
+    errs = {}
+    for prec in (U.PREC_MIXED, U.PREC_FP32):
+        with U.System(n, prec, eps2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+            assert s.tile()[1].startswith("sym")
+            s.set_state(c["pos"], c["vel"])
+            k0, w0, _ = s.diagnostics()
+            s.step(steps)
+            gp, _ = s.get_state()
+            k1, w1, mom = s.diagnostics()
+        errs[prec] = relnorm(gp[:, :3], p64[:, :3])
+        if prec == U.PREC_MIXED:
+            drift = abs((k1 + w1) - (k0 + w0)) / abs(k0 + w0)
+            say(f"  mixed: energy drift {drift:.3e}, |momentum| {np.abs(mom).max():.3e}")
+            assert drift <= 1e-6
+            assert np.abs(mom).max() <= 1e-8
+    say(f"  final-position rel err vs FP64: mixed {errs[U.PREC_MIXED]:.3e}, fp32 {errs[U.PREC_FP32]:.3e}")
+    assert errs[U.PREC_MIXED] <= 1e-6
+    assert errs[U.PREC_FP32] <= 1e-6
