@@ -175,7 +175,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8192)
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched"], default=None,
                     help="time one path only and print its line (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 256 MiB device copy (to scale HBM counters) and exit")
@@ -207,6 +207,12 @@ def main():
     log(f"device {name} CUs={cus} clock={clk} kHz; n={n} steps={K} warmup={W}")
 
     pairs = float(n) * float(n)
+    if args.only == "batched":
+        r = batched_config5(U)
+        print(json.dumps({"metric": METRIC, "value": r["graph_fwd_interactions_per_s"],
+                          "unit": "interactions/s", "n_gpus": 1, "only": "batched", "batched": r}),
+              flush=True)
+        return
     if args.only == "adjoint":
         r = fwd_adjoint(U, c, n, K, None)
         print(json.dumps({"metric": METRIC, "value": r["forward_equivalent_interactions_per_s"],

@@ -6,7 +6,8 @@
 // concatenated device state and one persistent launch per step (batch.hip);
 // the sequence of launches of a run is captured once in a HIP graph and
 // replayed.  Mode 1 keeps the per-system structure (one launch of the
-// single-system kernels per system and step) as the baseline.
+// single-system kernels per system and step, spread over four streams like
+// the reference's stream per task) as the baseline.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,6 +31,10 @@ constexpr int kWs[3] = {4, 8, 16};  // waves per work item of the one-pass kerne
 // dispatcher to place (5).
 constexpr int kSymPersist = 3, kSymFlat = 4, kSels = 5;
 constexpr int kSymUnitsPerItem = 2;  // work units (K source blocks of 64) per symmetric item
+// The per-system baseline (mode 1) spreads its launches over this many streams,
+// as the reference runs one task with its own stream per system; 4 is the
+// number of hardware queues a process gets by default.
+constexpr int kAuxStreams = 4;
 
 template <typename T>
 int dalloc(T** p, size_t count)
@@ -104,6 +109,8 @@ struct utcnb_batch {
     int g_nsteps = -1, g_ckpt = -1, g_w = -1;
     bool g_adj = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t aux[kAuxStreams] = {};  // per-system baseline streams (created on first use)
+    hipEvent_t evfork = nullptr, evjoin[kAuxStreams] = {};
 
     BatchGeom geom(int wi) const
     {
@@ -172,6 +179,17 @@ void release(utcnb_batch* b)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
         (void)hipEventDestroy(b->ev1);
+    for (int k = 0; k < kAuxStreams; ++k) {
+        if (b->aux[k])
+            (void)hipStreamDestroy(b->aux[k]);
+        if (b->evjoin[k])
+            (void)hipEventDestroy(b->evjoin[k]);
+        b->aux[k] = nullptr;
+        b->evjoin[k] = nullptr;
+    }
+    if (b->evfork)
+        (void)hipEventDestroy(b->evfork);
+    b->evfork = nullptr;
     if (b->own_stream && b->stream)
         (void)hipStreamDestroy(b->stream);
 }
@@ -356,6 +374,38 @@ int per_system_tile(int npad)
     return ids[best];
 }
 
+hipError_t aux_streams(utcnb_batch* b)
+{
+    if (b->aux[0])
+        return hipSuccess;
+    hipError_t e = hipEventCreateWithFlags(&b->evfork, hipEventDisableTiming);
+    for (int k = 0; k < kAuxStreams && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&b->aux[k], hipStreamNonBlocking);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&b->evjoin[k], hipEventDisableTiming);
+    }
+    return e;
+}
+
+// Launch fn(s, stream) for every system, system s on auxiliary stream
+// s mod kAuxStreams, after the work already on b->stream and before what
+// follows it (fork and join through events).
+template <class F>
+hipError_t per_system(utcnb_batch* b, F fn)
+{
+    hipError_t e = hipEventRecord(b->evfork, b->stream);
+    for (int k = 0; k < kAuxStreams && e == hipSuccess; ++k)
+        e = hipStreamWaitEvent(b->aux[k], b->evfork, 0);
+    for (int s = 0; s < b->nsys && e == hipSuccess; ++s)
+        e = fn(s, b->aux[s % kAuxStreams]);
+    for (int k = 0; k < kAuxStreams && e == hipSuccess; ++k) {
+        e = hipEventRecord(b->evjoin[k], b->aux[k]);
+        if (e == hipSuccess)
+            e = hipStreamWaitEvent(b->stream, b->evjoin[k], 0);
+    }
+    return e;
+}
+
 // Enqueue one run (forward nsteps, optionally the adjoint) on b->stream.
 // mode 0: batched kernels (one-pass with W = kWs[wi], or symmetric for
 // wi >= kSymPersist); mode 1: per-system launches.
@@ -382,7 +432,7 @@ hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode,
         if (mode == 0)
             return launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], so, sn, p, v, eps2, dt,
                                      damp, kick, st);
-        for (int s = 0; s < b->nsys; ++s) {
+        return per_system(b, [&](int s, hipStream_t ss) {
             StepArgs a{};
             a.n = b->n[s];
             a.n_pad = b->npad[s];
@@ -394,11 +444,8 @@ hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode,
             a.newton = false;
             a.tile = per_system_tile(b->npad[s]);
             const size_t o = (size_t)b->off[s];
-            hipError_t r = launch_mixed_step(a, so + o, sn + o, p + o, v + o, st);
-            if (r != hipSuccess)
-                return r;
-        }
-        return hipSuccess;
+            return launch_mixed_step(a, so + o, sn + o, p + o, v + o, ss);
+        });
     };
     auto vjp = [&]() -> hipError_t {
         if (mode == 0 && wi >= kSymPersist)
@@ -407,14 +454,11 @@ hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode,
         if (mode == 0)
             return launch_batch_vjp(kWs[wi], b->geom(wi), b->nblocks[wi], b->jrec, eps2, b->xbar,
                                     b->mbar, st);
-        for (int s = 0; s < b->nsys; ++s) {
+        return per_system(b, [&](int s, hipStream_t ss) {
             const size_t o = (size_t)b->off[s];
-            hipError_t r = launch_vjp_f32(b->jrec + 2 * o, b->n[s], b->npad[s], b->cfg.eps2,
-                                          b->xbar + 3 * o, b->mbar + o, b->cu_count, st);
-            if (r != hipSuccess)
-                return r;
-        }
-        return hipSuccess;
+            return launch_vjp_f32(b->jrec + 2 * o, b->n[s], b->npad[s], b->cfg.eps2, b->xbar + 3 * o,
+                                  b->mbar + o, b->cu_count, ss);
+        });
     };
 
     // every run starts from the FP64 state, so a replayed graph is consistent
@@ -752,6 +796,8 @@ int utcnb_batch_run(utcnb_batch* b, int nsteps, int mode, int ckpt_every, const 
         UTCNB_HIP(hipEventRecord(b->ev1, b->stream));
     } else {
         // mode 1: per-system launches; mode 2: the persistent kernels, eagerly
+        if (mode == 1)
+            UTCNB_HIP(aux_streams(b));
         UTCNB_HIP(hipEventRecord(b->ev0, b->stream));
         hipError_t e = enqueue_run(b, nsteps, adj ? ckpt_every : nsteps, adj, mode == 2 ? 0 : 1, wi);
         if (e != hipSuccess)
