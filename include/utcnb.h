@@ -153,7 +153,8 @@ int utcnb_adjoint(utcnb_system* sys, int nsteps, int ckpt_every, const double* t
 int utcnb_last_kernel_ms(utcnb_system* sys, double* ms, int* launches);
 
 /* Force-kernel tile id (1..utcnb_tile_count()) the handle runs: the configured
- * one, or the built-in default for its size when utcnb_config.tile was 0.
+ * one, or the built-in default for its size when utcnb_config.tile was 0; 0
+ * for the REFERENCE and FP64 precisions, whose kernels have no tiles.
  * No reference counterpart (the reference's block size is a launch argument). */
 int utcnb_get_tile(const utcnb_system* sys, int* tile);
 

@@ -138,6 +138,7 @@ def test_fp64_field_matches_cpu_fp64():
     say("case: FP64 field n=2048")
     a64 = O.accel_f64(c["pos"], c["eps"] ** 2)
     with U.System(2048, U.PREC_FP64, c["eps"] ** 2, c["dt"]) as s:
+        assert s.tile() == (0, "")  # the FP64 kernel has no tiles
         s.set_state(c["pos"], c["vel"])
         a = s.accel()
     assert relnorm(a, a64) <= 1e-14

@@ -203,10 +203,11 @@ class System:
         return ms.value, cnt.value
 
     def tile(self):
-        """(tile id, tile name) of the force kernel this system runs."""
+        """(tile id, tile name) of the force kernel this system runs; (0, "")
+        for the REFERENCE and FP64 precisions."""
         t = C.c_int()
         check(lib().utcnb_get_tile(self.h, C.byref(t)), "utcnb_get_tile")
-        return t.value, tiles()[t.value - 1]
+        return t.value, (tiles()[t.value - 1] if t.value > 0 else "")
 
     def adjoint(self, nsteps: int, ckpt_every: int, target3: np.ndarray):
         n = self.n

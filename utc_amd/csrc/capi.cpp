@@ -562,7 +562,8 @@ int utcnb_get_tile(const utcnb_system* s, int* tile)
 {
     if (!s || !tile)
         return fail(UTCNB_EINVAL, "utcnb_get_tile: null argument");
-    *tile = s->args.tile + 1;
+    const bool tiled = s->cfg.precision == UTCNB_PREC_FP32 || s->cfg.precision == UTCNB_PREC_MIXED;
+    *tile = tiled ? s->args.tile + 1 : 0;
     return UTCNB_OK;
 }
 
