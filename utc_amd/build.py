@@ -138,12 +138,23 @@ def build_oracle() -> None:
     _run([make, "-s", "-C", str(ROOT / "oracle")])
 
 
+def build_probes() -> list[Path]:
+    """Stand-alone gfx950 microbenchmarks (utc_amd/bench/*.hip) into utc_amd/lib/."""
+    outs = []
+    for src in sorted((PKG / "bench").glob("*.hip")):
+        exe = OUT / src.stem
+        _run([HIPCC, *HIP_FLAGS, "-o", str(exe), str(src)])
+        outs.append(exe)
+    return outs
+
+
 def build_all() -> None:
     with cf.ThreadPoolExecutor(max_workers=2) as ex:
         f_oracle = ex.submit(build_oracle)
         lib = build_library()
         f_oracle.result()
     build_host_programs()
+    build_probes()
     print(f"built {lib}")
 
 
