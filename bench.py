@@ -87,7 +87,7 @@ def cpu_baseline(c, n, threads, rows):
     t0 = time.perf_counter()
     O.gravitation_rows_f32(pos32, eps2, 0, rows, threads)
     dt = time.perf_counter() - t0
-    return {
+    out = {
         "value": rows * float(n) / dt,
         "unit": "interactions/s",
         "cores": threads,
@@ -96,6 +96,23 @@ def cpu_baseline(c, n, threads, rows):
                   f"(FP32, reference order) = {rows * float(n):.3g} pair interactions, "
                   f"{dt:.2f} s wall on {threads} threads",
     }
+    # the reference's own sequential code (benchapps/nbody/seq/bodysystem.cc,
+    # compiled from its sources into oracle/_ref), single-threaded as written
+    if O.REF_LIB.exists():
+        import ctypes as C
+        m = 16384
+        p = np.ascontiguousarray(pos32[:m])
+        v = np.zeros((m, 4), np.float32)
+        v[:, 3] = 1.0
+        t0 = time.perf_counter()
+        O.ref_lib().ref_run_f32(m, 1, p.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p),
+                                float(eps2), 1.0, 2.5e-4)
+        dt1 = time.perf_counter() - t0
+        out["reference_single_thread"] = {
+            "value": float(m) * m / dt1, "unit": "interactions/s", "cores": 1, "kind": "reference",
+            "sample": f"one step of the reference's BodySystem<float> on the first {m} bodies "
+                      f"({float(m) * m:.3g} pairs), {dt1:.2f} s"}
+    return out
 
 
 def batched_config5(U, nsys=256, steps=20, seed=0):
