@@ -76,6 +76,37 @@ def read_traffic(path: Path):
     return float(m.group(1)) if m else None
 
 
+def config2_accuracy(U):
+    """N = 65536 Plummer, eps = 0.005, dt = 5e-4, 20 steps: plain FP32 and mixed
+    against the on-device FP64 path (acceleration and final positions)."""
+    from tests.nbcases import config2
+    c = config2()
+    n, eps2, steps = len(c["pos"]), c["eps"] ** 2, c["steps"]
+    with U.System(n, U.PREC_FP64, eps2, c["dt"]) as s64:
+        s64.set_state(c["pos"], c["vel"])
+        a64 = s64.accel()
+        s64.step(steps)
+        p64, _ = s64.get_state()
+    out = {"workload": f"N={n} Plummer, eps=0.005, dt=5e-4, {steps} steps, physical kick"}
+    for label, prec in (("mixed", U.PREC_MIXED), ("fp32", U.PREC_FP32)):
+        with U.System(n, prec, eps2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+            s.set_state(c["pos"], c["vel"])
+            a = s.accel()
+            s.set_state(c["pos"], c["vel"])
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.sync()
+            wall = (time.perf_counter() - t0) * 1e3
+            p, _ = s.get_state()
+            out[label] = dict(tile=s.tile()[1], ms_per_step=wall / steps,
+                              interactions_per_s=float(n) * n * steps / (wall * 1e-3),
+                              accel_relerr=relnorm(a, a64), final_pos_relerr=relnorm(p[:, :3], p64[:, :3]))
+        log(f"config 2 {label}: {out[label]['ms_per_step']:.3f} ms/step, accel rel err "
+            f"{out[label]['accel_relerr']:.3e}, final-position rel err {out[label]['final_pos_relerr']:.3e}")
+    return out
+
+
 def cpu_baseline(c, n, threads, rows):
     """The oracle's restatement of benchapps/nbody/seq/bodysystem.cc:103-133
     (bit-identical to the reference), rows [0, rows) x all n sources, one
@@ -100,7 +131,7 @@ def cpu_baseline(c, n, threads, rows):
     # compiled from its sources into oracle/_ref), single-threaded as written
     if O.REF_LIB.exists():
         import ctypes as C
-        m = 16384
+        m = 65536  # ~10 s single-threaded
         p = np.ascontiguousarray(pos32[:m])
         v = np.zeros((m, 4), np.float32)
         v[:, 3] = 1.0
@@ -190,7 +221,8 @@ def main():
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=8192)
+    ap.add_argument("--cpu-rows", type=int, default=262144,
+                    help="rows of the CPU baseline's force evaluation (default: all, ~10 s on 16 threads)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched"], default=None,
                     help="time one path only and print its line (for profiler runs)")
@@ -293,6 +325,14 @@ def main():
                   momentum_end=[float(x) for x in p1])
     log(f"energy drift {energy['rel_drift']:.3e}, virial 2K/|W| {energy['virial_2K_over_W']:.4f}")
 
+    # ---- config 2 accuracy (N = 65536) ----------------------------------------------
+    cfg2 = None
+    try:
+        cfg2 = config2_accuracy(U)
+    except Exception as e:  # reported, never hidden
+        cfg2 = {"error": str(e)}
+        log(f"config 2 failed: {e}")
+
     # ---- forward + adjoint ----------------------------------------------------------
     fwd_adj = None
     if not args.no_adjoint:
@@ -367,6 +407,7 @@ def main():
         },
         "variants": results,
         "accuracy_vs_fp64": acc,
+        "config2_vs_fp64": cfg2,
         "energy": energy,
         "fwd_adjoint": fwd_adj,
         "batched": batch,
