@@ -121,7 +121,8 @@ struct utcnb_batch {
         return BatchSymGeom{d_sitems[k], d_sstart[k], d_off, d_n, d_nw, d_rs, d_ipo, d_jpo, d_vipo, d_vjpo,
                             d_sysblk};
     }
-    bool has_sym() const { return align == kBatchSymAlign; }
+    bool sym_ok = false;  // the symmetric kernels' geometry and force partials are set up
+    bool has_sym() const { return sym_ok; }
 };
 
 namespace {
@@ -310,10 +311,20 @@ hipError_t upload(T** d, const std::vector<T>& h)
     return hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
 }
 
+// Partial sums of the symmetric kernels are kept under this many bytes (force
+// and VJP each); batches above it use the one-pass kernels.
+constexpr size_t kSymPartialLimit = (size_t)16 << 30;
+
+// Sets up the symmetric kernels; returns hipSuccess with b->sym_ok false when
+// their partial sums would exceed kSymPartialLimit.
 hipError_t setup_sym(utcnb_batch* b)
 {
     SymPlan P;
     plan_sym(b, b->cu_count * 16, P);
+    const size_t fbytes = (size_t)(P.ifloats + P.jfloats) * sizeof(float);
+    const size_t vbytes = (size_t)(P.vi4 + P.vj4) * sizeof(float4);
+    if (fbytes > kSymPartialLimit || vbytes > kSymPartialLimit)
+        return hipSuccess;
     hipError_t e = upload(&b->d_nw, P.nw);
     if (e == hipSuccess) e = upload(&b->d_rs, P.rs);
     if (e == hipSuccess) e = upload(&b->d_ipo, P.ipo);
@@ -331,6 +342,7 @@ hipError_t setup_sym(utcnb_batch* b)
         e = hipErrorOutOfMemory;
     b->svi_count = (size_t)P.vi4;
     b->svj_count = (size_t)P.vj4;
+    b->sym_ok = e == hipSuccess;
     return e;
 }
 
@@ -592,15 +604,18 @@ int utcnb_batch_create(int nsys, const int32_t* sizes, const utcnb_config* cfg, 
             if (e == hipSuccess)
                 e = hipMemcpy(b->d_start[wi], start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice);
         }
-        if (e == hipSuccess && b->has_sym())
+        if (e == hipSuccess && b->align == kBatchSymAlign)
             e = setup_sym(b);
+        if (e == hipSuccess && cfg->tile > kSymPersist && !b->sym_ok)
+            rc = fail(UTCNB_EINVAL, "utcnb_batch_create: the symmetric batched kernels' partial sums "
+                                    "would exceed 16 GiB for these systems; use tile 0..3");
         if (e == hipSuccess)
             e = launch_batch_pad(b->geom(0), nsys, b->pos, b->vel, b->snap[0], b->snap[1], b->stream);
         if (e == hipSuccess && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess))
             e = hipErrorUnknown;
         if (e == hipSuccess)
             e = hipStreamSynchronize(b->stream);
-        if (e != hipSuccess)
+        if (e != hipSuccess && !rc)
             rc = fail(UTCNB_EHIP, std::string("utcnb_batch_create: ") + hipGetErrorString(e));
     }
     if (cfg->tile > 0)
