@@ -233,8 +233,8 @@ void schedule(const utcnb_batch* b, int W, int blocks, std::vector<int2>& items,
 }
 
 // Symmetric kernels: per system NW = npad / kBatchSymAlign wave blocks, each
-// with NW/2 + 1 work units split over R waves (about kSymUnitsPerItem units
-// each).  Items (system, wave block, split, R) are scheduled longest first,
+// with K (NW/2 + 1) (unit, source block) sub-units split over R waves (about
+// kSymUnitsPerItem units each); an item's cost is its number of sub-units.  Items (system, wave block, split, R) are scheduled longest first,
 // either onto `waves` persistent waves by least load, or one wave each.
 struct SymPlan {
     std::vector<int> nw, rs, sysblk;
@@ -253,8 +253,9 @@ void plan_sym(const utcnb_batch* b, int persistent_waves, SymPlan& P)
     std::vector<It> all;
     for (int s = 0; s < b->nsys; ++s) {
         const int NW = b->npad[s] / kBatchSymAlign;
-        const int units = NW / 2 + 1;
-        const int R = std::max(1, (units + kSymUnitsPerItem - 1) / kSymUnitsPerItem);
+        const int sub = (NW / 2 + 1) * kBatchSymK;  // (unit, source block) sub-units per wave block
+        const int per = kSymUnitsPerItem * kBatchSymK;
+        const int R = std::max(1, (sub + per - 1) / per);
         P.nw.push_back(NW);
         P.rs.push_back(R);
         P.ipo.push_back(P.ifloats);
@@ -269,9 +270,8 @@ void plan_sym(const utcnb_batch* b, int persistent_waves, SymPlan& P)
             P.sysblk.push_back(s);
         for (int w = 0; w < NW; ++w)
             for (int r = 0; r < R; ++r) {
-                const int u0 = (int)((long long)units * r / R), u1 = (int)((long long)units * (r + 1) / R);
-                if (u1 > u0)
-                    all.push_back({u1 - u0, make_int4(s, w, r, R)});
+                const int q0 = (int)((long long)sub * r / R), q1 = (int)((long long)sub * (r + 1) / R);
+                all.push_back({q1 - q0, make_int4(s, w, r, R)});
             }
     }
     std::stable_sort(all.begin(), all.end(), [](const It& a, const It& c) { return a.cost > c.cost; });

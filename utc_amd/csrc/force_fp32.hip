@@ -13,22 +13,25 @@ const Tile kTiles[] = {
     // symmetric (sym.hip): K targets per lane, R waves per wave block
     {4, 4, "sym4r4", true}, {4, 8, "sym4r8", true}, {8, 4, "sym8r4", true},
     {8, 8, "sym8r8", true}, {8, 16, "sym8r16", true}, {4, 16, "sym4r16", true},
-    {4, 32, "sym4r32", true}, {8, 32, "sym8r32", true},
+    {4, 32, "sym4r32", true}, {8, 32, "sym8r32", true}, {4, 64, "sym4r64", true},
+    {4, 128, "sym4r128", true}, {2, 32, "sym2r32", true}, {2, 64, "sym2r64", true},
 };
 }  // namespace
 
 int tile_count() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 const Tile& tile_get(int id) { return kTiles[id]; }
 
-// Default tile, from sweeps of every tile on MI355X (mixed precision, ms/step):
-//   N = 262144: sym8r32 14.33, sym8r16 14.54, k2s8 20.7 (best one-pass)
-//   N = 131072: sym8r32 3.73,  k2s8 5.30
-//   N =  65536: sym4r32 1.058, sym8r32 1.066, k2s8 1.40
-//   N =  32768: sym4r32 0.320, k2s8 0.426
-//   N =  16384: sym4r32 0.115, k1s16 0.117
-//   N =   8192: k1s16 0.061,  sym4r32 0.062
-// The symmetric tiles need enough wave blocks to fill the chip (K = 8 from 512
-// bodies per CU, K = 4 from 64), and their reaction partials must fit the
+// Default tile, from sweeps of the tiles on MI355X (mixed precision, ms/step,
+// best three; Plummer spheres):
+//   N = 262144: sym8r32 14.24, sym8r16 14.50, sym8r8 15.02   (k2s8, one-pass: 20.7)
+//   N = 131072: sym8r32 3.651, sym8r16 3.773, sym8r8 3.937
+//   N =  65536: sym8r32 0.974, sym8r16 1.020, sym4r64 1.052
+//   N =  32768: sym8r32 0.287, sym4r32 0.302, sym4r16 0.311
+//   N =  16384: sym4r32 0.103, sym4r64 0.109, sym8r32 0.116  (k1s16: 0.116)
+//   N =   8192: sym4r32 0.051, sym2r32 0.052, sym4r64 0.056  (k1s16: 0.061)
+//   N =   4096: sym2r32 0.031, k1s16 0.032
+// The symmetric tiles need enough wave blocks to fill the chip (K = 8 from 128
+// bodies per CU, K = 4 from 32), and their reaction partials must fit the
 // 16 GiB workspace limit; above it the best one-pass tile (k2s8) is used.
 int tile_default(int64_t n, int cu_count)
 {
@@ -38,9 +41,9 @@ int tile_default(int64_t n, int cu_count)
         return n_pad <= (1 << 30) && n_pad % (128 * k) == 0 &&
                sym_jpart_floats((int)n_pad, k) * sizeof(float) <= ((size_t)16 << 30);
     };
-    if (n >= 512 * cus && fits(8))
+    if (n >= 128 * cus && fits(8))
         return 17;  // sym8r32
-    if (n >= 64 * cus && fits(4))
+    if (n >= 32 * cus && fits(4))
         return 16;  // sym4r32
     if ((n + 127) / 128 * 8 >= cus * 16)
         return 8;   // k2s8

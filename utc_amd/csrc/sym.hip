@@ -16,9 +16,10 @@
 // sources sits one per lane and rotates around the wave with DPP wave_ror:1
 // for 64 steps, carrying its reaction accumulator, so after 64 steps every
 // target has met every source and each reaction sum is back in its home lane.
-// R waves split a wave block's work: unit 0 is the own blocks, unit 1 + d the
-// wave block d + 1 places round; wave r of the block takes units
-// [U*r/R, U*(r+1)/R) of U = NW/2 + 1.  Their target-side partials and the
+// R waves split a wave block's work into sub-units (u, kb): source block kb of
+// unit u, where unit 0 is the wave block's own sources and unit 1 + d the wave
+// block d + 1 places round; wave r takes sub-units [Q*r/R, Q*(r+1)/R) of
+// Q = K (NW/2 + 1).  Their target-side partials and the
 // reaction partials go to HBM and a second kernel adds them per body in a
 // fixed order (deterministic) and applies the integrator epilogue.
 #include "force_tmpl.h"
@@ -52,11 +53,14 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
         xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.w;
         ax[k] = ay[k] = az[k] = cx[k] = cy[k] = cz[k] = 0.f;
     }
-    const int units = half + 1;
-    const int u0 = (int)((long long)units * r / R), u1 = (int)((long long)units * (r + 1) / R);
-    if (u0 == 0 && u1 > 0) {
-        // own blocks: target side only (each ordered pair once, self pair gives 0)
-        for (int kb = 0; kb < K; ++kb) {
+    // sub-unit q = (unit u, source block kb): u = 0 is the wave block's own
+    // sources, u = 1 + d the wave block d + 1 places round
+    const int Q = (half + 1) * K;
+    const int q0 = (int)((long long)Q * r / R), q1 = (int)((long long)Q * (r + 1) / R);
+    for (int q = q0; q < q1; ++q) {
+        const int u = q / K, kb = q - u * K;
+        if (u == 0) {
+            // own block kb: target side only (each ordered pair once, self pair gives 0)
             const float4 s = src[w * 64 * K + kb * 64 + lane];
             float sx = s.x, sy = s.y, sz = s.z, sm = s.w;
             float px[K], py[K], pz[K];
@@ -82,52 +86,49 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
                 kahan_add(ay[k], cy[k], py[k]);
                 kahan_add(az[k], cz[k], pz[k]);
             }
-        }
-    }
-    const int d0 = u0 > 0 ? u0 - 1 : 0, d1 = u1 - 1;
-    for (int d = d0; d < d1; ++d) {
-        if (2 * (d + 1) == NW && w >= half)
             continue;
+        }
+        const int d = u - 1;
+        if (2 * (d + 1) == NW && w >= half)
+            continue;  // the block half way round: taken by the lower half
         int v = w + 1 + d;
         if (v >= NW)
             v -= NW;
-        for (int kb = 0; kb < K; ++kb) {
-            const float4 s = src[v * 64 * K + kb * 64 + lane];
-            float sx = s.x, sy = s.y, sz = s.z, sm = s.w, gx = 0.f, gy = 0.f, gz = 0.f;
-            float px[K], py[K], pz[K];
+        const float4 s = src[v * 64 * K + kb * 64 + lane];
+        float sx = s.x, sy = s.y, sz = s.z, sm = s.w, gx = 0.f, gy = 0.f, gz = 0.f;
+        float px[K], py[K], pz[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                px[k] = py[k] = pz[k] = 0.f;
+        for (int k = 0; k < K; ++k)
+            px[k] = py[k] = pz[k] = 0.f;
 #pragma unroll 4
-            for (int t = 0; t < 64; ++t) {
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
-                    const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-                    const float y3 = pair_scale<NEWTON>(d2, 1.0f);
-                    const float si = sm * y3;
-                    const float cj = mi[k] * y3;
-                    px[k] = __builtin_fmaf(rx, si, px[k]);
-                    py[k] = __builtin_fmaf(ry, si, py[k]);
-                    pz[k] = __builtin_fmaf(rz, si, pz[k]);
-                    gx = __builtin_fmaf(-rx, cj, gx);
-                    gy = __builtin_fmaf(-ry, cj, gy);
-                    gz = __builtin_fmaf(-rz, cj, gz);
-                }
-                sx = wave_ror1(sx); sy = wave_ror1(sy); sz = wave_ror1(sz); sm = wave_ror1(sm);
-                gx = wave_ror1(gx); gy = wave_ror1(gy); gz = wave_ror1(gz);
-            }
+        for (int t = 0; t < 64; ++t) {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                kahan_add(ax[k], cx[k], px[k]);
-                kahan_add(ay[k], cy[k], py[k]);
-                kahan_add(az[k], cz[k], pz[k]);
+                const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+                const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+                const float y3 = pair_scale<NEWTON>(d2, 1.0f);
+                const float si = sm * y3;
+                const float cj = mi[k] * y3;
+                px[k] = __builtin_fmaf(rx, si, px[k]);
+                py[k] = __builtin_fmaf(ry, si, py[k]);
+                pz[k] = __builtin_fmaf(rz, si, pz[k]);
+                gx = __builtin_fmaf(-rx, cj, gx);
+                gy = __builtin_fmaf(-ry, cj, gy);
+                gz = __builtin_fmaf(-rz, cj, gz);
             }
-            const size_t o = (((size_t)w * half + d) * (64 * K) + kb * 64 + lane) * 3;
-            jpart[o + 0] = gx;
-            jpart[o + 1] = gy;
-            jpart[o + 2] = gz;
+            sx = wave_ror1(sx); sy = wave_ror1(sy); sz = wave_ror1(sz); sm = wave_ror1(sm);
+            gx = wave_ror1(gx); gy = wave_ror1(gy); gz = wave_ror1(gz);
         }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            kahan_add(ax[k], cx[k], px[k]);
+            kahan_add(ay[k], cy[k], py[k]);
+            kahan_add(az[k], cz[k], pz[k]);
+        }
+        const size_t o = (((size_t)w * half + d) * (64 * K) + kb * 64 + lane) * 3;
+        jpart[o + 0] = gx;
+        jpart[o + 1] = gy;
+        jpart[o + 2] = gz;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -255,7 +256,12 @@ hipError_t launch_sym(const Tile& t, bool newton, const float4* src, int n, int 
     const int NW = n_pad / (64 * K);
     const int waves = NW * R;
     const dim3 grid((waves + 3) / 4), block(256);
-    if (K == 4) {
+    if (K == 2) {
+        if (newton)
+            hipLaunchKernelGGL((sym_force_kernel<2, true>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
+        else
+            hipLaunchKernelGGL((sym_force_kernel<2, false>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
+    } else if (K == 4) {
         if (newton)
             hipLaunchKernelGGL((sym_force_kernel<4, true>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
         else

@@ -16,7 +16,8 @@
 //
 // The mapping is the symmetric force kernel's (sym.hip): wave blocks of 64*K
 // bodies, unit 0 = own blocks (target side only), unit 1 + d = the wave block
-// d + 1 places round (both sides), R waves per wave block split the units.
+// d + 1 places round (both sides), R waves per wave block split the K (NW/2+1)
+// (unit, source block) sub-units.
 // Partials go to HBM (float4 per body: x, y, z, m) and a second kernel adds
 // them per body in FP64, in a fixed order, into xbar and mbar.
 #include "internal.h"
@@ -69,11 +70,14 @@ __device__ __forceinline__ void sym_vjp_wave(const float4* __restrict__ jrec, in
         for (int q = 0; q < 4; ++q)
             g[k][q] = c[k][q] = 0.f;
     }
-    const int units = half + 1;
-    const int u0 = (int)((long long)units * r / R), u1 = (int)((long long)units * (r + 1) / R);
-    if (u0 == 0 && u1 > 0) {
-        // own blocks: target side only; they hold the self pairs, so w is exact (vjp_pair.h)
-        for (int kb = 0; kb < K; ++kb) {
+    // sub-unit su = (unit u, source block kb): u = 0 is the wave block's own
+    // sources, u = 1 + d the wave block d + 1 places round
+    const int Q = (half + 1) * K;
+    const int q0 = (int)((long long)Q * r / R), q1 = (int)((long long)Q * (r + 1) / R);
+    for (int su = q0; su < q1; ++su) {
+        const int u = su / K, kb = su - u * K;
+        if (u == 0) {
+            // own block kb: target side only; it holds the self pairs, so w is exact (vjp_pair.h)
             const int b = w * 64 * K + kb * 64 + lane;
             const float4 sp = jrec[2 * b], sa = jrec[2 * b + 1];
             float sx = sp.x, sy = sp.y, sz = sp.z, sm = sp.w, ax = sa.x, ay = sa.y, az = sa.z;
@@ -107,59 +111,56 @@ __device__ __forceinline__ void sym_vjp_wave(const float4* __restrict__ jrec, in
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     kahan1(g[k][q], c[k][q], p[k][q]);
-        }
-    }
-    const int d0 = u0 > 0 ? u0 - 1 : 0, d1 = u1 - 1;
-    for (int d = d0; d < d1; ++d) {
-        if (2 * (d + 1) == NW && w >= half)
             continue;
+        }
+        const int d = u - 1;
+        if (2 * (d + 1) == NW && w >= half)
+            continue;  // the block half way round: taken by the lower half
         int v = w + 1 + d;
         if (v >= NW)
             v -= NW;
-        for (int kb = 0; kb < K; ++kb) {
-            const int b = v * 64 * K + kb * 64 + lane;
-            const float4 sp = jrec[2 * b], sa = jrec[2 * b + 1];
-            float sx = sp.x, sy = sp.y, sz = sp.z, sm = sp.w, ax = sa.x, ay = sa.y, az = sa.z;
-            float hx = 0.f, hy = 0.f, hz = 0.f, hm = 0.f;  // reaction on the source body
-            float p[K][4];
+        const int b = v * 64 * K + kb * 64 + lane;
+        const float4 sp = jrec[2 * b], sa = jrec[2 * b + 1];
+        float sx = sp.x, sy = sp.y, sz = sp.z, sm = sp.w, ax = sa.x, ay = sa.y, az = sa.z;
+        float hx = 0.f, hy = 0.f, hz = 0.f, hm = 0.f;  // reaction on the source body
+        float p[K][4];
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
+        for (int k = 0; k < K; ++k)
+            p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
 #pragma unroll 2
-            for (int t = 0; t < 64; ++t) {
+        for (int t = 0; t < 64; ++t) {
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
-                    const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-                    const float y = __builtin_amdgcn_rsqf(d2);
-                    const float y2 = y * y, y3 = y2 * y;
-                    // J w = y^3 (w - 3 y^2 (r.w) r), r.w = m_i (a_j.r) - m_j (a_i.r)
-                    const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
-                    const float ari = __builtin_fmaf(bx[k], rx, __builtin_fmaf(by[k], ry, bz[k] * rz));
-                    const float s3 = (-3.f * y2) * __builtin_fmaf(mi[k], arj, -(sm * ari));
-                    const float tx = __builtin_fmaf(s3, rx, w_fast(mi[k], ax, sm, bx[k]));
-                    const float ty = __builtin_fmaf(s3, ry, w_fast(mi[k], ay, sm, by[k]));
-                    const float tz = __builtin_fmaf(s3, rz, w_fast(mi[k], az, sm, bz[k]));
-                    p[k][0] = __builtin_fmaf(y3, tx, p[k][0]);
-                    p[k][1] = __builtin_fmaf(y3, ty, p[k][1]);
-                    p[k][2] = __builtin_fmaf(y3, tz, p[k][2]);
-                    hx = __builtin_fmaf(-y3, tx, hx);
-                    hy = __builtin_fmaf(-y3, ty, hy);
-                    hz = __builtin_fmaf(-y3, tz, hz);
-                    p[k][3] = __builtin_fmaf(-y3, arj, p[k][3]);
-                    hm = __builtin_fmaf(y3, ari, hm);
-                }
-                sx = ror1(sx); sy = ror1(sy); sz = ror1(sz); sm = ror1(sm);
-                ax = ror1(ax); ay = ror1(ay); az = ror1(az);
-                hx = ror1(hx); hy = ror1(hy); hz = ror1(hz); hm = ror1(hm);
+            for (int k = 0; k < K; ++k) {
+                const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+                const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+                const float y = __builtin_amdgcn_rsqf(d2);
+                const float y2 = y * y, y3 = y2 * y;
+                // J w = y^3 (w - 3 y^2 (r.w) r), r.w = m_i (a_j.r) - m_j (a_i.r)
+                const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
+                const float ari = __builtin_fmaf(bx[k], rx, __builtin_fmaf(by[k], ry, bz[k] * rz));
+                const float s3 = (-3.f * y2) * __builtin_fmaf(mi[k], arj, -(sm * ari));
+                const float tx = __builtin_fmaf(s3, rx, w_fast(mi[k], ax, sm, bx[k]));
+                const float ty = __builtin_fmaf(s3, ry, w_fast(mi[k], ay, sm, by[k]));
+                const float tz = __builtin_fmaf(s3, rz, w_fast(mi[k], az, sm, bz[k]));
+                p[k][0] = __builtin_fmaf(y3, tx, p[k][0]);
+                p[k][1] = __builtin_fmaf(y3, ty, p[k][1]);
+                p[k][2] = __builtin_fmaf(y3, tz, p[k][2]);
+                hx = __builtin_fmaf(-y3, tx, hx);
+                hy = __builtin_fmaf(-y3, ty, hy);
+                hz = __builtin_fmaf(-y3, tz, hz);
+                p[k][3] = __builtin_fmaf(-y3, arj, p[k][3]);
+                hm = __builtin_fmaf(y3, ari, hm);
             }
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    kahan1(g[k][q], c[k][q], p[k][q]);
-            jpart[((size_t)w * half + d) * (64 * K) + kb * 64 + lane] = make_float4(hx, hy, hz, hm);
+            sx = ror1(sx); sy = ror1(sy); sz = ror1(sz); sm = ror1(sm);
+            ax = ror1(ax); ay = ror1(ay); az = ror1(az);
+            hx = ror1(hx); hy = ror1(hy); hz = ror1(hz); hm = ror1(hm);
         }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                kahan1(g[k][q], c[k][q], p[k][q]);
+        jpart[((size_t)w * half + d) * (64 * K) + kb * 64 + lane] = make_float4(hx, hy, hz, hm);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -268,15 +269,15 @@ constexpr int kSymVjpK = 4;
 
 int sym_vjp_splits(int n_pad, int cu_count)
 {
-    // ~16 waves per SIMD, at most one unit per wave, at most 32 splits
+    // ~16 waves per SIMD, at least one source block per wave, at most 64 splits
     const int NW = n_pad / (64 * kSymVjpK);
     const int want = (cu_count > 0 ? cu_count : 256) * 4 * 16;
     int R = (want + NW - 1) / NW;
-    const int units = NW / 2 + 1;
-    if (R > units)
-        R = units;
-    if (R > 32)
-        R = 32;
+    const int sub = (NW / 2 + 1) * kSymVjpK;
+    if (R > sub)
+        R = sub;
+    if (R > 64)
+        R = 64;
     return R < 1 ? 1 : R;
 }
 
