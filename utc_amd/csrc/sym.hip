@@ -8,10 +8,10 @@
 // rotation step, 7 wave-wide DPP moves shared by the K targets of a lane:
 // (16 + 7/K)/2 slots per ordered pair, 8.4 at K = 8.
 //
-// Mapping.  Bodies form wave blocks of 64*K (lane l holds K targets).  Wave w
-// takes (a) its own K blocks of 64 sources, target side only, and (b) the
-// source blocks of wave blocks v = w+1 .. w+NW/2 (mod NW), both sides; for even
-// NW the block half way round is taken by the lower half only, so every
+// Mapping.  Bodies form wave blocks of 64*K (lane l holds K targets).  Wave
+// block w takes (a) its own K blocks of 64 sources, target side only, and (b)
+// the source blocks of wave blocks v = w+1 .. w+NW/2 (mod NW), both sides; for
+// even NW the block half way round is taken by the lower half only, so every
 // unordered pair of wave blocks is visited exactly once.  A block of 64
 // sources sits one per lane and rotates around the wave with DPP wave_ror:1
 // for 64 steps, carrying its reaction accumulator, so after 64 steps every
@@ -19,9 +19,9 @@
 // R waves split a wave block's work into sub-units (u, kb): source block kb of
 // unit u, where unit 0 is the wave block's own sources and unit 1 + d the wave
 // block d + 1 places round; wave r takes sub-units [Q*r/R, Q*(r+1)/R) of
-// Q = K (NW/2 + 1).  Their target-side partials and the
-// reaction partials go to HBM and a second kernel adds them per body in a
-// fixed order (deterministic) and applies the integrator epilogue.
+// Q = K (NW/2 + 1).  The waves' target-side partials and the reaction
+// partials go to HBM, and a second kernel adds them per body in a fixed order
+// (deterministic) and applies the integrator epilogue.
 #include "force_tmpl.h"
 
 namespace utcnb {
