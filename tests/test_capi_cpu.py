@@ -138,3 +138,11 @@ def test_create_without_gpu_reports_no_device():
     cfg = U.Config(10, U.PREC_MIXED, 1, 1, 0, 0, 1e-4, 1.0, 1e-3)
     assert L.utcnb_create(C.byref(cfg), None, C.byref(h)) == U.ENODEV
     assert b"device" in L.utcnb_last_error()
+
+
+def test_autotune_validates_before_touching_a_device():
+    L = U.lib()
+    t, ms = C.c_int(), C.c_double()
+    assert L.utcnb_autotune(0, U.PREC_MIXED, None, C.byref(t), C.byref(ms)) == U.EINVAL
+    assert L.utcnb_autotune(100, U.PREC_FP64, None, C.byref(t), C.byref(ms)) == U.EINVAL
+    assert L.utcnb_autotune(100, U.PREC_MIXED, None, None, None) == U.EINVAL

@@ -236,3 +236,29 @@ def test_full_size_accuracy_against_fp64(which):
     say(f"  final-position rel err vs FP64: mixed {errs[U.PREC_MIXED]:.3e}, fp32 {errs[U.PREC_FP32]:.3e}")
     assert errs[U.PREC_MIXED] <= 1e-6
     assert errs[U.PREC_FP32] <= 1e-6
+
+
+def test_autotune_picks_a_valid_tile_and_caches_it_by_name(tmp_path):
+    """utcnb_autotune times every tile for the size class (N rounded up to 1024),
+    returns a tile usable in utcnb_config.tile, and keeps the result in memory
+    and in the cache file (one line: arch, precision, class, tile name, ms)."""
+    cache = tmp_path / "tiles.txt"
+    n = 5000
+    say(f"case: autotune n={n}")
+    tile, ms = U.autotune(n, U.PREC_MIXED, cache)
+    assert 1 <= tile <= U.lib().utcnb_tile_count() and ms > 0
+    fields = cache.read_text().split()
+    assert fields[0].startswith("gfx950") and fields[1] == str(U.PREC_MIXED)
+    assert fields[2] == "5120" and fields[3] == U.tiles()[tile - 1]
+    assert U.autotune(n, U.PREC_MIXED, cache) == (tile, ms)  # from the cache
+    # a file from a build whose tile table differs: names, not ids, are kept
+    cache.write_text(f"{fields[0]} {U.PREC_FP32} 5120 no-such-tile 1.0\n"
+                     f"{fields[0]} {U.PREC_FP32} 6144 k1s8 0.5\n")
+    t2, ms2 = U.autotune(6000, U.PREC_FP32, cache)
+    assert (t2, ms2) == (U.tiles().index("k1s8") + 1, 0.5)
+    c = plummer(n, eps=0.005)
+    a64 = O.accel_f64(c["pos"], c["eps"] ** 2)
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], tile=tile) as s:
+        s.set_state(c["pos"], c["vel"])
+        a = s.accel()
+    assert relnorm(a, a64) <= 2e-6

@@ -228,6 +228,15 @@ def device_info():
     return name.value.decode(), cu.value, clk.value
 
 
+def autotune(n: int, precision: int = PREC_MIXED, cache_path=None):
+    """(tile id, ms per step) of the fastest force-kernel tile for n bodies
+    (utcnb_autotune); cache_path: optional text cache file."""
+    t, ms = C.c_int(), C.c_double()
+    path = None if cache_path is None else str(cache_path).encode()
+    check(lib().utcnb_autotune(n, precision, path, C.byref(t), C.byref(ms)), "utcnb_autotune")
+    return t.value, ms.value
+
+
 def tiles():
     L = lib()
     return [L.utcnb_tile_name(t).decode() for t in range(1, L.utcnb_tile_count() + 1)]

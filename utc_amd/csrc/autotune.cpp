@@ -3,8 +3,10 @@
 //
 // Results are cached per (device, precision, size class) in memory and,
 // optionally, in a text file with one line per entry:
-//   <device-arch> <precision> <n> <tile-id> <ms-per-step>
-// The size class is n rounded up to the source padding (1024 bodies).
+//   <device-arch> <precision> <n> <tile-name> <ms-per-step>
+// The size class is n rounded up to the source padding (1024 bodies).  Tiles
+// are stored by name, so a file written by a build with another tile table
+// maps onto this one; lines naming a tile this build lacks are ignored.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -34,17 +36,28 @@ std::string device_arch()
     return c == std::string::npos ? a : a.substr(0, c);
 }
 
+int tile_by_name(const char* name)
+{
+    for (int t = 0; t < utcnb::tile_count(); ++t)
+        if (std::strcmp(utcnb::tile_get(t).name, name) == 0)
+            return t + 1;
+    return 0;
+}
+
 void load_file(const char* path)
 {
     FILE* f = std::fopen(path, "r");
     if (!f)
         return;
-    char arch[64];
-    int prec, tile;
+    char arch[64], name[64];
+    int prec;
     long long n;
     double ms;
-    while (std::fscanf(f, "%63s %d %lld %d %lf", arch, &prec, &n, &tile, &ms) == 5)
-        g_cache[Key(arch, prec, n)] = {tile, ms};
+    while (std::fscanf(f, "%63s %d %lld %63s %lf", arch, &prec, &n, name, &ms) == 5) {
+        const int tile = tile_by_name(name);
+        if (tile > 0)
+            g_cache[Key(arch, prec, n)] = {tile, ms};
+    }
     std::fclose(f);
 }
 
@@ -55,9 +68,9 @@ void save_file(const char* path)
     if (!f)
         return;
     for (auto& kv : g_cache)
-        std::fprintf(f, "%s %d %lld %d %.6f\n", std::get<0>(kv.first).c_str(),
-                     std::get<1>(kv.first), std::get<2>(kv.first), kv.second.first,
-                     kv.second.second);
+        std::fprintf(f, "%s %d %lld %s %.17g\n", std::get<0>(kv.first).c_str(),
+                     std::get<1>(kv.first), std::get<2>(kv.first),
+                     utcnb::tile_get(kv.second.first - 1).name, kv.second.second);
     std::fclose(f);
     std::rename(tmp.c_str(), path);
 }
@@ -127,6 +140,8 @@ extern "C" int utcnb_autotune(int n, int precision, const char* cache_path, int*
             best = t;
         }
     }
+    if (best == 0)
+        return fail(UTCNB_EHIP, "utcnb_autotune: no tile could be timed");
     g_cache[Key(arch, precision, cls)] = {best, bms};
     if (cache_path)
         save_file(cache_path);
