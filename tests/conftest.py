@@ -22,7 +22,7 @@ def _has_gpu() -> bool:
 
 
 def pytest_collection_modifyitems(config, items):
-    # GPU tests fail loudly (not skip) when selected with -m gpu on a GPU box;
+    # GPU tests fail loudly (not skip) when selected with -m gpu on a machine with a GPU;
     # without a GPU they are skipped so the CPU suite stays green.
     if _has_gpu():
         return
