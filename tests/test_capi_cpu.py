@@ -126,6 +126,7 @@ def test_create_validates_before_touching_a_device():
         assert L.utcnb_last_error()
     assert L.utcnb_integrate_f32(None, None, None, 10, 0.1, 0.1, 1.0, 256, None) == U.EINVAL
     assert L.utcnb_generate(U.GEN_CUBE, 0, 0, 0, 0, 0, None, None, None) == U.EINVAL
+    assert b"utcnb_generate" in L.utcnb_last_error()
     assert L.utcnb_generate(99, 10, 0, 0, 0, 0, None, None, None) == U.EINVAL
 
 

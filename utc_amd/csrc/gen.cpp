@@ -18,9 +18,14 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "utcnb.h"
+
+namespace utcnb {
+int fail(int code, const std::string& msg);  // capi.cpp: records utcnb_last_error()
+}
 
 namespace {
 
@@ -95,20 +100,20 @@ int utcnb_generate(int kind, int n, uint64_t seed, double p0, double p1, double 
                    double* pos4, double* vel4, double* out3)
 {
     if (n <= 0)
-        return UTCNB_EINVAL;
+        return utcnb::fail(UTCNB_EINVAL, "utcnb_generate: n must be >= 1");
     Stream g(seed);
     if (kind == UTCNB_GEN_TARGETS) {
         if (!out3)
-            return UTCNB_EINVAL;
+            return utcnb::fail(UTCNB_EINVAL, "utcnb_generate: targets need out3");
         for (int64_t i = 0; i < 3 * (int64_t)n; ++i)
             out3[i] = p0 * g.normal();
         return UTCNB_OK;
     }
     if (!pos4 || !vel4)
-        return UTCNB_EINVAL;
+        return utcnb::fail(UTCNB_EINVAL, "utcnb_generate: pos4 and vel4 are required");
     if (kind == UTCNB_GEN_CUBE || kind == UTCNB_GEN_CUBE_LOGM) {
         if (kind == UTCNB_GEN_CUBE_LOGM && !(p1 > 0 && p2 >= p1))
-            return UTCNB_EINVAL;
+            return utcnb::fail(UTCNB_EINVAL, "utcnb_generate: log-uniform masses need 0 < p1 <= p2");
         for (int i = 0; i < n; ++i) {
             for (int k = 0; k < 3; ++k)
                 pos4[4 * (size_t)i + k] = 2.0 * g.uniform() - 1.0;
@@ -162,7 +167,7 @@ int utcnb_generate(int kind, int n, uint64_t seed, double p0, double p1, double 
             }
         return UTCNB_OK;
     }
-    return UTCNB_EINVAL;
+    return utcnb::fail(UTCNB_EINVAL, "utcnb_generate: unknown kind");
 }
 
 }  // extern "C"
