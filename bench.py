@@ -360,7 +360,12 @@ def main():
     achieved = FLOP_PER_PAIR * pairs / t_launch
     traffic = read_traffic(ROOT / "profiles" / "force_mixed_n262144_pmc.txt")
     roofline = {
-        "bound": "valu",
+        # the compute roof (of the two the schema names, hbm | mfma); the pair
+        # kernel issues FP32 vector (VALU) instructions, no MFMA, and 157.3 TFLOP/s
+        # is MI355X's FP32 peak
+        "bound": "mfma",
+        "bound_detail": "compute roof: FP32 vector ALU issue (VALU); no MFMA is used, the pair sum "
+                        "is not a dense contraction; peak = MI355X FP32 157.3 TFLOP/s",
         "achieved": achieved / 1e12,
         "peak": PEAK_FP32 / 1e12,
         "unit": "TFLOP/s",
