@@ -49,6 +49,13 @@ def relnorm(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def device_sync(s):
+    """The system's stream, then the whole device (torch.cuda.synchronize)."""
+    import torch
+    s.sync()
+    torch.cuda.synchronize()
+
+
 def timed_steps(U, prec, n, c, steps, warmup, flags=0, tile=0):
     """Run warmup + timed steps; returns (wall_ms, kernel_ms, launches, system)."""
     s = U.System(n, prec, c["eps"] ** 2, c["dt"], flags=flags, tile=tile)
@@ -58,10 +65,10 @@ def timed_steps(U, prec, n, c, steps, warmup, flags=0, tile=0):
     s.sync()
     # reset to the initial state so the timed steps are the configuration's steps
     s.set_state(c["pos"], c["vel"])
-    s.sync()
+    device_sync(s)
     t0 = time.perf_counter()
     s.step(steps)
-    s.sync()
+    device_sync(s)
     wall = (time.perf_counter() - t0) * 1e3
     kms, launches = s.last_kernel_ms()
     return wall, kms, launches, s
