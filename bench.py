@@ -16,7 +16,7 @@ drift, forward+adjoint throughput, the roofline of the force kernel and a
 CPU baseline (the oracle's restatement of the reference's sequential CPU path,
 threaded over bodies with the reference's partition rule, on a row sample).
 
-Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--n 262144]
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 5] [--n 262144]
 """
 from __future__ import annotations
 
@@ -224,7 +224,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true")
