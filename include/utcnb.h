@@ -113,7 +113,8 @@ int utcnb_destroy(utcnb_system* sys);
  * system's state type).  where = UTCNB_HOST for host pointers, UTCNB_DEVICE for
  * device pointers.  Replaces GpuData::initH + sync (H2D) at
  * benchapps/nbody/utc/sgpu/body_task_sgpu.cu:102-111.  vel4 may be NULL
- * (zero velocities, w = 1).
+ * (zero velocities, w = 1).  Synchronizes the system's stream when
+ * where = UTCNB_HOST, so host buffers may be reused on return.
  */
 int utcnb_set_state(utcnb_system* sys, const void* pos4, const void* vel4, int dtype,
                     int where);

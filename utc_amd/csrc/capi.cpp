@@ -461,8 +461,9 @@ int utcnb_set_state(utcnb_system* s, const void* pos4, const void* vel4, int dty
     }
     if (rc)
         return rc;
-    // the host default buffer must outlive the async copy
-    if (!vdefault.empty())
+    // host sources (the caller's, or the default velocities) may be reused or
+    // freed once this returns: wait for the copies out of them
+    if (where == UTCNB_HOST || !vdefault.empty())
         UTCNB_HIP(hipStreamSynchronize(s->stream));
     s->has_state = true;
     return UTCNB_OK;
