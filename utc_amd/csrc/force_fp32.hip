@@ -15,6 +15,7 @@ const Tile kTiles[] = {
     {8, 8, "sym8r8", true}, {8, 16, "sym8r16", true}, {4, 16, "sym4r16", true},
     {4, 32, "sym4r32", true}, {8, 32, "sym8r32", true}, {4, 64, "sym4r64", true},
     {4, 128, "sym4r128", true}, {2, 32, "sym2r32", true}, {2, 64, "sym2r64", true},
+    {8, 64, "sym8r64", true}, {8, 128, "sym8r128", true},
 };
 }  // namespace
 
@@ -30,6 +31,9 @@ const Tile& tile_get(int id) { return kTiles[id]; }
 //   N =  16384: sym4r32 0.103, sym4r64 0.109, sym8r32 0.116  (k1s16: 0.116)
 //   N =   8192: sym4r32 0.051, sym2r32 0.052, sym4r64 0.056  (k1s16: 0.061)
 //   N =   4096: sym2r32 0.031, k1s16 0.032
+// (sym8r64, added later, runs within 0.5% of sym8r32 from N = 32768: 14.15 vs
+// 14.21 ms at N = 262144, 0.970 vs 0.974 at 65536 -- inside run-to-run noise,
+// so the default stays sym8r32 with half the target-side workspace.)
 // The symmetric tiles need enough wave blocks to fill the chip (K = 8 from 128
 // bodies per CU, K = 4 from 32), and their reaction partials must fit the
 // 16 GiB workspace limit; above it the best one-pass tile (k2s8) is used.
