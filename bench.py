@@ -254,8 +254,15 @@ def main():
               "path is design-only in this build.", file=sys.stderr)
         sys.exit(2)
 
+    import torch
+
     import utc_amd as U
     from tests.nbcases import plummer
+
+    # bring up torch's device context before anything is timed (device_sync
+    # uses it; its first use must not fall inside a timed region)
+    torch.zeros(1, device="cuda")
+    torch.cuda.synchronize()
 
     n, K, W = args.n, args.steps, args.warmup
     c = plummer(n, eps=0.002, dt=2.5e-4, steps=K)
