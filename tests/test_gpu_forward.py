@@ -262,3 +262,26 @@ def test_autotune_picks_a_valid_tile_and_caches_it_by_name(tmp_path):
         s.set_state(c["pos"], c["vel"])
         a = s.accel()
     assert relnorm(a, a64) <= 2e-6
+
+
+def test_long_run_mixed_tracks_fp64():
+    """500 steps (half a crossing time) of a 16384-body Plummer sphere, eps = 0.01,
+    dt = 1e-3: the mixed path (default symmetric tile) stays within 1e-6 of the
+    FP64 path in final positions, and both conserve energy to 1e-5."""
+    n, steps = 16384, 500
+    c = plummer(n, eps=0.01, dt=1e-3)
+    say(f"case: long run n={n} steps={steps}")
+    out = {}
+    for prec in (U.PREC_FP64, U.PREC_MIXED):
+        with U.System(n, prec, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+            s.set_state(c["pos"], c["vel"])
+            k0, w0, _ = s.diagnostics()
+            s.step(steps)
+            p, _ = s.get_state()
+            k1, w1, _ = s.diagnostics()
+        out[prec] = (p, abs((k1 + w1) - (k0 + w0)) / abs(k0 + w0))
+    err = relnorm(out[U.PREC_MIXED][0][:, :3], out[U.PREC_FP64][0][:, :3])
+    say(f"  final-position rel err {err:.3e}, energy drift fp64 {out[U.PREC_FP64][1]:.3e} "
+        f"mixed {out[U.PREC_MIXED][1]:.3e}")
+    assert err <= 1e-6
+    assert out[U.PREC_FP64][1] <= 1e-5 and out[U.PREC_MIXED][1] <= 1e-5
