@@ -285,3 +285,22 @@ def test_long_run_mixed_tracks_fp64():
         f"mixed {out[U.PREC_MIXED][1]:.3e}")
     assert err <= 1e-6
     assert out[U.PREC_FP64][1] <= 1e-5 and out[U.PREC_MIXED][1] <= 1e-5
+
+
+def test_set_state_host_buffers_reusable_on_return():
+    """utcnb_set_state from host memory returns after its copies: overwriting
+    the caller's arrays right after the call does not change the state."""
+    n = 4096
+    c = plummer(n)
+    pos, vel = c["pos"].copy(), c["vel"].copy()
+    say("case: set_state host buffer reuse")
+    for prec in (U.PREC_MIXED, U.PREC_FP32, U.PREC_FP64):
+        with U.System(n, prec, c["eps"] ** 2, c["dt"]) as s:
+            s.set_state(pos, vel)
+            pos[:] = 7.0
+            vel[:] = -7.0
+            gp, gv = s.get_state()
+            pos[:], vel[:] = c["pos"], c["vel"]
+        tol = 0.0 if prec == U.PREC_FP64 else 1e-6
+        assert np.abs(gp - c["pos"]).max() <= tol * np.abs(c["pos"]).max()
+        assert np.abs(gv - c["vel"]).max() <= tol * np.abs(c["vel"]).max() + tol
