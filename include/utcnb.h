@@ -183,7 +183,8 @@ int utcnb_batch_get_state(utcnb_batch* b, double* pos4, double* vel4);
  * cfg->tile of the batch selects the batched kernels: 1..3 = one-pass with 4,
  * 8 or 16 waves per (system, 64-target tile) work item on persistent
  * workgroups; 4 = symmetric (each unordered pair once) with (system, wave
- * block, split) items on persistent waves; 5 = symmetric, one wave per item;
+ * block, split) items, longest first, taken by persistent waves through an
+ * atomic work counter; 5 = symmetric, one wave per item;
  * 0 = time all five on the first mode 0/2 run and keep the fastest.  With
  * tile 0, 4 or 5 every system is padded to a multiple of 256 bodies, else 64.
  * The batched adjoint supports the physical kick only.

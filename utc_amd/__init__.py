@@ -283,7 +283,7 @@ class Batch:
         return p, v
 
     BATCH_TILES = {0: "not tuned yet", 1: "one-pass, 4 waves per item", 2: "one-pass, 8 waves per item",
-                   3: "one-pass, 16 waves per item", 4: "symmetric, persistent waves",
+                   3: "one-pass, 16 waves per item", 4: "symmetric, persistent waves on a work counter",
                    5: "symmetric, one wave per item"}
 
     def tile(self):

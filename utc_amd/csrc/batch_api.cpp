@@ -26,9 +26,9 @@ namespace {
 
 constexpr int kWs[3] = {4, 8, 16};  // waves per work item of the one-pass kernels (batch tiles 1..3)
 // Batch tiles 4 and 5: the symmetric kernels (sym.hip, sym_vjp.hip), work items
-// (system, wave block, split) on persistent waves in a longest-first static
-// schedule (4), or one wave per item, longest first, for the hardware
-// dispatcher to place (5).
+// (system, wave block, split) in a longest-first list, taken by persistent
+// waves through an atomic work counter (4), or one wave per item for the
+// hardware dispatcher to place (5).
 constexpr int kSymPersist = 3, kSymFlat = 4, kSels = 5;
 constexpr int kSymUnitsPerItem = 2;  // work units (K source blocks of 64) per symmetric item
 // The per-system baseline (mode 1) spreads its launches over this many streams,
@@ -89,6 +89,8 @@ struct utcnb_batch {
     int4* d_sitems[2] = {nullptr, nullptr};
     int* d_sstart[2] = {nullptr, nullptr};
     int snwaves[2] = {0, 0};
+    int snitems = 0;
+    int* d_counter = nullptr;  // [2]: work counters of the persistent step and VJP kernels
     float* sipart = nullptr;
     float* sjpart = nullptr;
     size_t svi_count = 0, svj_count = 0;  // VJP partials (float4), allocated with the adjoint workspace
@@ -116,10 +118,11 @@ struct utcnb_batch {
     {
         return BatchGeom{d_off, d_n, d_chunks, d_items[wi], d_start[wi]};
     }
-    BatchSymGeom sym_geom(int k) const
+    // k: 0 persistent waves (dynamic), 1 one wave per item; which: 0 step, 1 VJP counter
+    BatchSymGeom sym_geom(int k, int which) const
     {
         return BatchSymGeom{d_sitems[k], d_sstart[k], d_off, d_n, d_nw, d_rs, d_ipo, d_jpo, d_vipo, d_vjpo,
-                            d_sysblk};
+                            d_sysblk, k == 0 ? d_counter + which : nullptr, snitems};
     }
     bool sym_ok = false;  // the symmetric kernels' geometry and force partials are set up
     bool has_sym() const { return sym_ok; }
@@ -176,6 +179,7 @@ void release(utcnb_batch* b)
     }
     dfree(b->sipart);
     dfree(b->sjpart);
+    dfree(b->d_counter);
     if (b->ev0)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
@@ -234,14 +238,17 @@ void schedule(const utcnb_batch* b, int W, int blocks, std::vector<int2>& items,
 
 // Symmetric kernels: per system NW = npad / kBatchSymAlign wave blocks, each
 // with K (NW/2 + 1) (unit, source block) sub-units split over R waves (about
-// kSymUnitsPerItem units each); an item's cost is its number of sub-units.  Items (system, wave block, split, R) are scheduled longest first,
-// either onto `waves` persistent waves by least load, or one wave each.
+// kSymUnitsPerItem units each); an item's cost is its number of sub-units.
+// Items (system, wave block, split, R) are listed longest first; persistent
+// waves take them in that order through an atomic work counter, or each item
+// gets a wave of its own.
 struct SymPlan {
     std::vector<int> nw, rs, sysblk;
     std::vector<long long> ipo, jpo, vipo, vjpo;
     long long ifloats = 0, jfloats = 0, vi4 = 0, vj4 = 0;
     std::vector<int4> items[2];
     std::vector<int> start[2];
+    int waves0 = 0;  // persistent waves
 };
 
 void plan_sym(const utcnb_batch* b, int persistent_waves, SymPlan& P)
@@ -275,32 +282,17 @@ void plan_sym(const utcnb_batch* b, int persistent_waves, SymPlan& P)
             }
     }
     std::stable_sort(all.begin(), all.end(), [](const It& a, const It& c) { return a.cost > c.cost; });
-    // one wave per item
+    // one wave per item (start[w] = w)
     P.items[1].clear();
     P.start[1].assign(1, 0);
     for (const It& it : all) {
         P.items[1].push_back(it.item);
         P.start[1].push_back((int)P.items[1].size());
     }
-    // persistent waves, least-loaded first
-    const int W = (int)std::max<size_t>(1, std::min<size_t>(all.size(), (size_t)persistent_waves));
-    using Load = std::pair<long long, int>;
-    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-    for (int k = 0; k < W; ++k)
-        heap.push({0, k});
-    std::vector<std::vector<int4>> per(W);
-    for (const It& it : all) {
-        Load l = heap.top();
-        heap.pop();
-        per[l.second].push_back(it.item);
-        heap.push({l.first + it.cost, l.second});
-    }
-    P.items[0].clear();
-    P.start[0].assign(1, 0);
-    for (int k = 0; k < W; ++k) {
-        P.items[0].insert(P.items[0].end(), per[k].begin(), per[k].end());
-        P.start[0].push_back((int)P.items[0].size());
-    }
+    // persistent waves take the same longest-first list through a work counter
+    P.items[0] = P.items[1];
+    P.start[0] = {0, (int)P.items[0].size()};
+    P.waves0 = (int)std::max<size_t>(1, std::min<size_t>(all.size(), (size_t)persistent_waves));
 }
 
 template <typename T>
@@ -336,8 +328,14 @@ hipError_t setup_sym(utcnb_batch* b)
         e = upload(&b->d_sitems[k], P.items[k]);
         if (e == hipSuccess)
             e = upload(&b->d_sstart[k], P.start[k]);
-        b->snwaves[k] = (int)P.start[k].size() - 1;
     }
+    b->snwaves[0] = P.waves0;
+    b->snwaves[1] = (int)P.items[1].size();
+    b->snitems = (int)P.items[1].size();
+    if (e == hipSuccess && dalloc(&b->d_counter, 2))
+        e = hipErrorOutOfMemory;
+    if (e == hipSuccess)
+        e = hipMemset(b->d_counter, 0, 2 * sizeof(int));
     if (e == hipSuccess && (dalloc(&b->sipart, (size_t)P.ifloats) || dalloc(&b->sjpart, (size_t)P.jfloats)))
         e = hipErrorOutOfMemory;
     b->svi_count = (size_t)P.vi4;
@@ -439,7 +437,7 @@ hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode,
     const int sk = wi - kSymPersist;  // symmetric schedule (0 persistent, 1 one wave per item)
     auto step = [&](const float4* so, float4* sn, double4* p, double4* v) -> hipError_t {
         if (mode == 0 && wi >= kSymPersist)
-            return launch_batch_sym_step(b->sym_geom(sk), b->snwaves[sk], (int)T, so, sn, p, v, eps2, dt,
+            return launch_batch_sym_step(b->sym_geom(sk, 0), b->snwaves[sk], (int)T, so, sn, p, v, eps2, dt,
                                          damp, kick, b->sipart, b->sjpart, st);
         if (mode == 0)
             return launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], so, sn, p, v, eps2, dt,
@@ -461,7 +459,7 @@ hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode,
     };
     auto vjp = [&]() -> hipError_t {
         if (mode == 0 && wi >= kSymPersist)
-            return launch_batch_sym_vjp(b->sym_geom(sk), b->snwaves[sk], (int)T, b->jrec, eps2, b->svipart,
+            return launch_batch_sym_vjp(b->sym_geom(sk, 1), b->snwaves[sk], (int)T, b->jrec, eps2, b->svipart,
                                         b->svjpart, b->xbar, b->mbar, st);
         if (mode == 0)
             return launch_batch_vjp(kWs[wi], b->geom(wi), b->nblocks[wi], b->jrec, eps2, b->xbar,
@@ -711,7 +709,7 @@ static int tune_w(utcnb_batch* b)
     const float eps2 = (float)b->cfg.eps2;
     auto one = [&](int wi) -> hipError_t {
         if (wi >= kSymPersist)
-            return launch_batch_sym_step(b->sym_geom(wi - kSymPersist), b->snwaves[wi - kSymPersist],
+            return launch_batch_sym_step(b->sym_geom(wi - kSymPersist, 0), b->snwaves[wi - kSymPersist],
                                          (int)T, b->snap[0], b->snap[1], b->pos, b->vel, eps2, b->cfg.dt,
                                          b->cfg.damping, b->cfg.kick, b->sipart, b->sjpart, b->stream);
         return launch_batch_step(kWs[wi], b->geom(wi), b->nblocks[wi], b->snap[0], b->snap[1], b->pos,

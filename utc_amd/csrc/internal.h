@@ -142,7 +142,10 @@ struct BatchSymGeom {
     const long long* vipo;    // VJP partials (float4 units), target side
     const long long* vjpo;    //   ... reactions
     const int* sys_of_block;  // system of each kBatchSymAlign-body block
+    int* counter;             // work counter of the dynamic schedule, or null (static lists)
+    int nitems;               // items in the dynamic schedule
 };
+hipError_t launch_batch_counter_reset(int* counter, hipStream_t st);
 hipError_t launch_batch_sym_step(const BatchSymGeom& g, int nwaves, int total, const float4* snap_old,
                                  float4* snap_new, double4* pos, double4* vel, float eps2, double dt,
                                  double damping, int kick, float* ipart, float* jpart, hipStream_t st);

@@ -151,8 +151,25 @@ __global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict
     sym_wave<K, NEWTON>(src, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
 }
 
-// Batched systems: wave gw runs the items [start[gw], start[gw+1]), each
-// (system, wave block, split) of a system laid out as for one system.
+// The next item of a wave.  Static schedule: the items [start[gw], start[gw+1])
+// in turn.  Dynamic schedule (g.counter set): lane 0 takes the next index of
+// the longest-first list with one atomic add; every wave stops at g.nitems,
+// so all waves end once the list is used up.
+__device__ __forceinline__ int batch_next_item(const BatchSymGeom& g, int gw, int& pos, int lane)
+{
+    if (!g.counter) {
+        const int it = g.start[gw] + pos++;
+        return it < g.start[gw + 1] ? it : -1;
+    }
+    int it = 0;
+    if (lane == 0)
+        it = atomicAdd(g.counter, 1);
+    it = __shfl(it, 0, 64);
+    return it < g.nitems ? it : -1;
+}
+
+// Batched systems: each wave runs items (system, wave block, split), a system
+// laid out as for one system.
 template <int K>
 __global__ __launch_bounds__(256) void batch_sym_kernel(BatchSymGeom g, int nwaves, const float4* __restrict__ snap,
                                                         float eps2, float* __restrict__ ipart,
@@ -162,12 +179,21 @@ __global__ __launch_bounds__(256) void batch_sym_kernel(BatchSymGeom g, int nwav
     if (gw >= nwaves)
         return;  // whole waves only
     const int lane = threadIdx.x & 63;
-    for (int it = g.start[gw]; it < g.start[gw + 1]; ++it) {
+    int pos = 0;
+    for (int it = batch_next_item(g, gw, pos, lane); it >= 0; it = batch_next_item(g, gw, pos, lane)) {
         const int4 item = g.items[it];  // (system, wave block, split, splits)
         const int s = item.x;
         sym_wave<K, false>(snap + g.off[s], g.nw[s], item.w, item.y, item.z, eps2, ipart + g.ipo[s],
                            jpart + g.jpo[s], lane);
     }
+}
+
+// Zeroes the dynamic schedule's work counter before a launch (a kernel, not a
+// memset, so the step can sit in a captured graph).
+__global__ void batch_counter_reset_kernel(int* counter)
+{
+    if (threadIdx.x == 0)
+        *counter = 0;
 }
 
 // Sum of the partials of body b (index within its system) in FP64, in a
@@ -282,13 +308,22 @@ hipError_t launch_sym(const Tile& t, bool newton, const float4* src, int n, int 
     return hipGetLastError();
 }
 
+hipError_t launch_batch_counter_reset(int* counter, hipStream_t st)
+{
+    hipLaunchKernelGGL(batch_counter_reset_kernel, dim3(1), dim3(64), 0, st, counter);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_sym_step(const BatchSymGeom& g, int nwaves, int total, const float4* snap_old,
                                  float4* snap_new, double4* pos, double4* vel, float eps2, double dt,
                                  double damping, int kick, float* ipart, float* jpart, hipStream_t st)
 {
+    hipError_t e;
+    if (g.counter && (e = launch_batch_counter_reset(g.counter, st)) != hipSuccess)
+        return e;
     hipLaunchKernelGGL((batch_sym_kernel<kBatchSymK>), dim3((nwaves + 3) / 4), dim3(256), 0, st, g, nwaves,
                        snap_old, eps2, ipart, jpart);
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     if (e != hipSuccess)
         return e;
     EpiMixed epi{pos, vel, snap_new, dt, damping, kick};

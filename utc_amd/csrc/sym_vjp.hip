@@ -180,7 +180,21 @@ __global__ __launch_bounds__(256) void sym_vjp_kernel(const float4* __restrict__
     sym_vjp_wave<K>(jrec, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
 }
 
-// Batched systems: wave gw runs the items [start[gw], start[gw+1]).
+// Batched systems: each wave runs items, in the schedule batch_next_item
+// describes (sym.hip).
+__device__ __forceinline__ int batch_vjp_next_item(const BatchSymGeom& g, int gw, int& pos, int lane)
+{
+    if (!g.counter) {
+        const int it = g.start[gw] + pos++;
+        return it < g.start[gw + 1] ? it : -1;
+    }
+    int it = 0;
+    if (lane == 0)
+        it = atomicAdd(g.counter, 1);
+    it = __shfl(it, 0, 64);
+    return it < g.nitems ? it : -1;
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void batch_sym_vjp_kernel(BatchSymGeom g, int nwaves,
                                                             const float4* __restrict__ jrec, float eps2,
@@ -191,7 +205,8 @@ __global__ __launch_bounds__(256) void batch_sym_vjp_kernel(BatchSymGeom g, int 
     if (gw >= nwaves)
         return;  // whole waves only
     const int lane = threadIdx.x & 63;
-    for (int it = g.start[gw]; it < g.start[gw + 1]; ++it) {
+    int pos = 0;
+    for (int it = batch_vjp_next_item(g, gw, pos, lane); it >= 0; it = batch_vjp_next_item(g, gw, pos, lane)) {
         const int4 item = g.items[it];  // (system, wave block, split, splits)
         const int s = item.x;
         sym_vjp_wave<K>(jrec + 2 * (size_t)g.off[s], g.nw[s], item.w, item.y, item.z, eps2,
@@ -311,9 +326,12 @@ hipError_t launch_batch_sym_vjp(const BatchSymGeom& g, int nwaves, int total, co
                                 hipStream_t st)
 {
     static_assert(kBatchSymK == kSymVjpK, "batched and single-system symmetric VJP share K");
+    hipError_t e;
+    if (g.counter && (e = launch_batch_counter_reset(g.counter, st)) != hipSuccess)
+        return e;
     hipLaunchKernelGGL((batch_sym_vjp_kernel<kBatchSymK>), dim3((nwaves + 3) / 4), dim3(256), 0, st, g,
                        nwaves, jrec, eps2, ipart, jpart);
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL(batch_sym_vjp_reduce_kernel, dim3((total + 255) / 256), dim3(256), 0, st, g, total,
