@@ -63,3 +63,23 @@ def relnorm(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def mass_pattern(pos, pattern, seed=5):
+    """Masses (total 1) for the symmetric kernels' two pair forms: blocks whose
+    targets and sources share one mass drop the mass factors."""
+    n = len(pos)
+    p = pos.copy()
+    rng = np.random.default_rng(seed)
+    if pattern == "equal":  # one mass, not a power of two: every block uniform
+        m = np.full(n, 1.0 / n)
+    elif pattern == "one_heavy":  # uniform except the blocks that hold body n/3
+        m = np.full(n, 1.0)
+        m[n // 3] = 50.0
+    elif pattern == "few_heavy":  # a scatter of different masses: mixed blocks
+        m = np.full(n, 1.0)
+        m[rng.choice(n, 12, replace=False)] = rng.uniform(2, 20, 12)
+    else:  # log-uniform: no uniform block
+        m = np.exp(rng.uniform(np.log(0.1), np.log(10.0), n))
+    p[:, 3] = m / m.sum()
+    return p

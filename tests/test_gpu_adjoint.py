@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import utc_amd as U
-from nbcases import config4, relnorm
+from nbcases import config4, mass_pattern, relnorm
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -193,6 +193,26 @@ def test_symmetric_vjp_matches_one_pass_and_fp64():
     e = [relnorm(res["sym"][q], res["onepass"][q]) for q in (1, 2, 3)]
     say(f"  sym vs onepass: dx {e[0]:.2e} dv {e[1]:.2e} dm {e[2]:.2e}")
     assert max(e) <= 1e-5
+
+
+@pytest.mark.parametrize("pattern", ["equal", "one_heavy", "few_heavy"])
+def test_symmetric_vjp_mass_patterns(pattern):
+    """Blocks whose targets and sources share one mass take the symmetric
+    VJP's mass-free pair form; a few differing masses leave the blocks that
+    hold them on the general form.  Both against the FP64 adjoint."""
+    n, steps, eps2, dt = 5000, 6, 1e-4, 1e-3
+    pos, vel, tgt = small_case(n, steps, seed=11)
+    pos = mass_pattern(pos, pattern)
+    res = {}
+    for label, prec, tile in (("fp64", U.PREC_FP64, 0), ("sym", U.PREC_MIXED, tile_id("sym4r32"))):
+        say(f"case: VJP {label} masses={pattern} n={n}")
+        with U.System(n, prec, eps2, dt, tile=tile) as s:
+            s.set_state(pos, vel)
+            res[label] = s.adjoint(steps, 3, tgt)
+    e = [relnorm(res["sym"][q], res["fp64"][q]) for q in (1, 2, 3)]
+    say(f"  sym vs fp64: dx {e[0]:.2e} dv {e[1]:.2e} dm {e[2]:.2e}")
+    assert max(e) <= 1e-5
+    assert res["sym"][0] == pytest.approx(res["fp64"][0], rel=1e-9)
 
 
 def test_autograd_function_wrapper():

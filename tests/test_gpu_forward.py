@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import utc_amd as U
-from nbcases import config1, config2, config3, plummer, relnorm
+from nbcases import config1, config2, config3, mass_pattern, plummer, relnorm
 from oracle import oracle as O
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -157,6 +157,23 @@ def test_every_tile_field_accuracy(prec, tile):
         a = s.accel()
     # the FP32 positions alone perturb close pairs by ~u|x|/r
     assert relnorm(a, a64) <= 2e-6
+
+
+@pytest.mark.parametrize("pattern", ["equal", "one_heavy", "few_heavy", "loguniform"])
+@pytest.mark.parametrize("name", ["sym2r32", "sym4r32", "sym8r32", "sym8r4"])
+def test_symmetric_tiles_mass_patterns(name, pattern):
+    n = 6000  # not a multiple of 512: the last wave block holds padding
+    c = plummer(n, eps=0.005)
+    pos = mass_pattern(c["pos"], pattern)
+    tile = U.tiles().index(name) + 1
+    a64 = O.accel_f64(pos, c["eps"] ** 2)
+    say(f"case: symmetric field tile={name} masses={pattern} n={n}")
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], tile=tile) as s:
+        s.set_state(pos, c["vel"])
+        a = s.accel()
+    assert relnorm(a, a64) <= 2e-6
+    # every body, not only the norm over all bodies
+    assert np.abs(a - a64).max() <= 1e-5 * np.abs(a64).max()
 
 
 @pytest.mark.parametrize("n", [1, 7, 64, 1000, 4097])

@@ -6,7 +6,12 @@
 // here an unordered pair costs 16 (3 sub, 3 fma, 2 mul for y^3, 2 mul for the
 // two mass factors, 6 fma for the two accumulations) plus, per 64-source
 // rotation step, 7 wave-wide DPP moves shared by the K targets of a lane:
-// (16 + 7/K)/2 slots per ordered pair, 8.4 at K = 8.
+// (16 + 7/K)/2 slots per ordered pair, 8.4 at K = 8.  When the wave's targets
+// and the block's sources all have one mass m (the reference's initial
+// conditions give every body mass 1), both sides use y^3 r directly and the
+// block partials are scaled by m once: 14 slots and 6 DPP moves, (14 + 6/K)/2
+// = 7.4 per ordered pair.  For m a power of two the two forms are bitwise
+// equal; otherwise they differ by the rounding of one product per partial.
 //
 // Mapping.  Bodies form wave blocks of 64*K (lane l holds K targets).  Wave
 // block w takes (a) its own K blocks of 64 sources, target side only, and (b)
@@ -36,6 +41,58 @@ __device__ __forceinline__ float wave_ror1(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x13C /* wave_ror:1 */, 0xF, 0xF, false));
 }
 
+// A block of 64 sources (one per lane, s) against the K targets of each lane,
+// both sides: the target-side block partials are Kahan-added into (a, c), the
+// reaction sums of the lane's source come back in g after 64 rotations.
+// UNI: every target mass mi[k] and source mass equals m, so the pair terms
+// drop the mass factors and the block sums are scaled by m at the end.
+template <int K, bool NEWTON, bool UNI>
+__device__ __forceinline__ void sym_block(float4 s, const float (&xi)[K], const float (&yi)[K],
+                                          const float (&zi)[K], const float (&mi)[K], float m, float eps2,
+                                          float (&ax)[K], float (&ay)[K], float (&az)[K], float (&cx)[K],
+                                          float (&cy)[K], float (&cz)[K], float& gx, float& gy, float& gz)
+{
+    float sx = s.x, sy = s.y, sz = s.z, sm = s.w;
+    float px[K], py[K], pz[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        px[k] = py[k] = pz[k] = 0.f;
+    gx = gy = gz = 0.f;
+#pragma unroll 4
+    for (int t = 0; t < 64; ++t) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+            const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+            const float y3 = pair_scale<NEWTON>(d2, 1.0f);
+            const float si = UNI ? y3 : sm * y3;
+            const float cj = UNI ? y3 : mi[k] * y3;
+            px[k] = __builtin_fmaf(rx, si, px[k]);
+            py[k] = __builtin_fmaf(ry, si, py[k]);
+            pz[k] = __builtin_fmaf(rz, si, pz[k]);
+            gx = __builtin_fmaf(-rx, cj, gx);
+            gy = __builtin_fmaf(-ry, cj, gy);
+            gz = __builtin_fmaf(-rz, cj, gz);
+        }
+        sx = wave_ror1(sx); sy = wave_ror1(sy); sz = wave_ror1(sz);
+        if constexpr (!UNI)
+            sm = wave_ror1(sm);
+        gx = wave_ror1(gx); gy = wave_ror1(gy); gz = wave_ror1(gz);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if constexpr (UNI) {
+            px[k] *= m; py[k] *= m; pz[k] *= m;
+        }
+        kahan_add(ax[k], cx[k], px[k]);
+        kahan_add(ay[k], cy[k], py[k]);
+        kahan_add(az[k], cz[k], pz[k]);
+    }
+    if constexpr (UNI) {
+        gx *= m; gy *= m; gz *= m;
+    }
+}
+
 // The work of wave r of wave block w of one system: src = the system's
 // padded bodies (NW wave blocks of 64K); ipart: [R][n_pad][3] target-side
 // sums; jpart: [NW][NW/2][64K][3] reactions.
@@ -53,6 +110,12 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
         xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.w;
         ax[k] = ay[k] = az[k] = cx[k] = cy[k] = cz[k] = 0.f;
     }
+    // one mass for every target of the wave?
+    const float m0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(mi[0])));
+    bool tuni = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        tuni = tuni && mi[k] == m0;
     // sub-unit q = (unit u, source block kb): u = 0 is the wave block's own
     // sources, u = 1 + d the wave block d + 1 places round
     const int Q = (half + 1) * K;
@@ -95,36 +158,11 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
         if (v >= NW)
             v -= NW;
         const float4 s = src[v * 64 * K + kb * 64 + lane];
-        float sx = s.x, sy = s.y, sz = s.z, sm = s.w, gx = 0.f, gy = 0.f, gz = 0.f;
-        float px[K], py[K], pz[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            px[k] = py[k] = pz[k] = 0.f;
-#pragma unroll 4
-        for (int t = 0; t < 64; ++t) {
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
-                const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-                const float y3 = pair_scale<NEWTON>(d2, 1.0f);
-                const float si = sm * y3;
-                const float cj = mi[k] * y3;
-                px[k] = __builtin_fmaf(rx, si, px[k]);
-                py[k] = __builtin_fmaf(ry, si, py[k]);
-                pz[k] = __builtin_fmaf(rz, si, pz[k]);
-                gx = __builtin_fmaf(-rx, cj, gx);
-                gy = __builtin_fmaf(-ry, cj, gy);
-                gz = __builtin_fmaf(-rz, cj, gz);
-            }
-            sx = wave_ror1(sx); sy = wave_ror1(sy); sz = wave_ror1(sz); sm = wave_ror1(sm);
-            gx = wave_ror1(gx); gy = wave_ror1(gy); gz = wave_ror1(gz);
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            kahan_add(ax[k], cx[k], px[k]);
-            kahan_add(ay[k], cy[k], py[k]);
-            kahan_add(az[k], cz[k], pz[k]);
-        }
+        float gx, gy, gz;
+        if (__ballot(tuni && s.w == m0) == ~0ull)
+            sym_block<K, NEWTON, true>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
+        else
+            sym_block<K, NEWTON, false>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
         const size_t o = (((size_t)w * half + d) * (64 * K) + kb * 64 + lane) * 3;
         jpart[o + 0] = gx;
         jpart[o + 1] = gy;

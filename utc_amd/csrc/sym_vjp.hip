@@ -12,7 +12,11 @@
 // products, 2 for r.w, 2 mul for -3 y^2 r.w, 6 for w, 3 fma for the bracket,
 // 6 fma for the two position terms, 2 fma for the two mass terms) against 31
 // per ordered pair in the one-pass VJP, plus 11 DPP moves per 64-source
-// rotation step shared by the K targets of a lane.
+// rotation step shared by the K targets of a lane.  When the targets of the
+// wave and the sources of a block all have one mass m, w = m (alpha_j -
+// alpha_i) and r.w = m (a_j.r - a_i.r): the pair works with the differences
+// alone and the block's position partials are scaled by m once (31 slots, 10
+// DPP moves).
 //
 // The mapping is the symmetric force kernel's (sym.hip): wave blocks of 64*K
 // bodies, unit 0 = own blocks (target side only), unit 1 + d = the wave block
@@ -48,6 +52,76 @@ __device__ __forceinline__ void kahan1(float& s, float& c, float p)
 
 }  // namespace
 
+// A block of 64 sources (one per lane: position/mass sp, alpha sa) against
+// the K targets of each lane, both sides: the target-side block partials are
+// Kahan-added into (g, c); h receives the source's reaction sums after 64
+// rotations.  UNI: every target mass mi[k] and source mass equals m.
+template <int K, bool UNI>
+__device__ __forceinline__ void sym_vjp_block(float4 sp, float4 sa, const float (&xi)[K], const float (&yi)[K],
+                                              const float (&zi)[K], const float (&mi)[K], const float (&bx)[K],
+                                              const float (&by)[K], const float (&bz)[K], float m, float eps2,
+                                              float (&g)[K][4], float (&c)[K][4], float4& h)
+{
+    float sx = sp.x, sy = sp.y, sz = sp.z, sm = sp.w, ax = sa.x, ay = sa.y, az = sa.z;
+    float hx = 0.f, hy = 0.f, hz = 0.f, hm = 0.f;  // reaction on the source body
+    float p[K][4];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
+#pragma unroll 2
+    for (int t = 0; t < 64; ++t) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+            const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+            const float y = __builtin_amdgcn_rsqf(d2);
+            const float y2 = y * y, y3 = y2 * y;
+            // J w = y^3 (w - 3 y^2 (r.w) r), r.w = m_i (a_j.r) - m_j (a_i.r)
+            const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
+            const float ari = __builtin_fmaf(bx[k], rx, __builtin_fmaf(by[k], ry, bz[k] * rz));
+            float s3, wx, wy, wz;
+            if constexpr (UNI) {  // w / m and r.w / m
+                s3 = (-3.f * y2) * (arj - ari);
+                wx = ax - bx[k]; wy = ay - by[k]; wz = az - bz[k];
+            } else {
+                s3 = (-3.f * y2) * __builtin_fmaf(mi[k], arj, -(sm * ari));
+                wx = w_fast(mi[k], ax, sm, bx[k]);
+                wy = w_fast(mi[k], ay, sm, by[k]);
+                wz = w_fast(mi[k], az, sm, bz[k]);
+            }
+            const float tx = __builtin_fmaf(s3, rx, wx);
+            const float ty = __builtin_fmaf(s3, ry, wy);
+            const float tz = __builtin_fmaf(s3, rz, wz);
+            p[k][0] = __builtin_fmaf(y3, tx, p[k][0]);
+            p[k][1] = __builtin_fmaf(y3, ty, p[k][1]);
+            p[k][2] = __builtin_fmaf(y3, tz, p[k][2]);
+            hx = __builtin_fmaf(-y3, tx, hx);
+            hy = __builtin_fmaf(-y3, ty, hy);
+            hz = __builtin_fmaf(-y3, tz, hz);
+            p[k][3] = __builtin_fmaf(-y3, arj, p[k][3]);
+            hm = __builtin_fmaf(y3, ari, hm);
+        }
+        sx = ror1(sx); sy = ror1(sy); sz = ror1(sz);
+        if constexpr (!UNI)
+            sm = ror1(sm);
+        ax = ror1(ax); ay = ror1(ay); az = ror1(az);
+        hx = ror1(hx); hy = ror1(hy); hz = ror1(hz); hm = ror1(hm);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if constexpr (UNI) {
+            p[k][0] *= m; p[k][1] *= m; p[k][2] *= m;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            kahan1(g[k][q], c[k][q], p[k][q]);
+    }
+    if constexpr (UNI) {
+        hx *= m; hy *= m; hz *= m;
+    }
+    h = make_float4(hx, hy, hz, hm);
+}
+
 // The work of wave r of wave block w of one system.  jrec: two float4 per
 // body {x,y,z,m} {ax,ay,az,0}, n_pad = NW*64*K bodies; ipart: [R][n_pad]
 // float4; jpart: [NW][NW/2][64K] float4.
@@ -70,6 +144,12 @@ __device__ __forceinline__ void sym_vjp_wave(const float4* __restrict__ jrec, in
         for (int q = 0; q < 4; ++q)
             g[k][q] = c[k][q] = 0.f;
     }
+    // one mass m0 for every target of the wave?
+    const float m0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(mi[0])));
+    bool tuni = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        tuni = tuni && mi[k] == m0;
     // sub-unit su = (unit u, source block kb): u = 0 is the wave block's own
     // sources, u = 1 + d the wave block d + 1 places round
     const int Q = (half + 1) * K;
@@ -121,46 +201,12 @@ __device__ __forceinline__ void sym_vjp_wave(const float4* __restrict__ jrec, in
             v -= NW;
         const int b = v * 64 * K + kb * 64 + lane;
         const float4 sp = jrec[2 * b], sa = jrec[2 * b + 1];
-        float sx = sp.x, sy = sp.y, sz = sp.z, sm = sp.w, ax = sa.x, ay = sa.y, az = sa.z;
-        float hx = 0.f, hy = 0.f, hz = 0.f, hm = 0.f;  // reaction on the source body
-        float p[K][4];
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
-#pragma unroll 2
-        for (int t = 0; t < 64; ++t) {
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
-                const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-                const float y = __builtin_amdgcn_rsqf(d2);
-                const float y2 = y * y, y3 = y2 * y;
-                // J w = y^3 (w - 3 y^2 (r.w) r), r.w = m_i (a_j.r) - m_j (a_i.r)
-                const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
-                const float ari = __builtin_fmaf(bx[k], rx, __builtin_fmaf(by[k], ry, bz[k] * rz));
-                const float s3 = (-3.f * y2) * __builtin_fmaf(mi[k], arj, -(sm * ari));
-                const float tx = __builtin_fmaf(s3, rx, w_fast(mi[k], ax, sm, bx[k]));
-                const float ty = __builtin_fmaf(s3, ry, w_fast(mi[k], ay, sm, by[k]));
-                const float tz = __builtin_fmaf(s3, rz, w_fast(mi[k], az, sm, bz[k]));
-                p[k][0] = __builtin_fmaf(y3, tx, p[k][0]);
-                p[k][1] = __builtin_fmaf(y3, ty, p[k][1]);
-                p[k][2] = __builtin_fmaf(y3, tz, p[k][2]);
-                hx = __builtin_fmaf(-y3, tx, hx);
-                hy = __builtin_fmaf(-y3, ty, hy);
-                hz = __builtin_fmaf(-y3, tz, hz);
-                p[k][3] = __builtin_fmaf(-y3, arj, p[k][3]);
-                hm = __builtin_fmaf(y3, ari, hm);
-            }
-            sx = ror1(sx); sy = ror1(sy); sz = ror1(sz); sm = ror1(sm);
-            ax = ror1(ax); ay = ror1(ay); az = ror1(az);
-            hx = ror1(hx); hy = ror1(hy); hz = ror1(hz); hm = ror1(hm);
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                kahan1(g[k][q], c[k][q], p[k][q]);
-        jpart[((size_t)w * half + d) * (64 * K) + kb * 64 + lane] = make_float4(hx, hy, hz, hm);
+        float4 h;
+        if (__ballot(tuni && sp.w == m0) == ~0ull)
+            sym_vjp_block<K, true>(sp, sa, xi, yi, zi, mi, bx, by, bz, m0, eps2, g, c, h);
+        else
+            sym_vjp_block<K, false>(sp, sa, xi, yi, zi, mi, bx, by, bz, m0, eps2, g, c, h);
+        jpart[((size_t)w * half + d) * (64 * K) + kb * 64 + lane] = h;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k)
