@@ -299,6 +299,21 @@ def main():
         log(f"{label} [{s.tile()[1]}]: {wall / K:.3f} ms/step wall, {kms / launches:.3f} ms/step kernel, "
             f"{results[label]['interactions_per_s']:.4e} interactions/s")
 
+    # the same positions with log-uniform masses (0.1..10): no block has one
+    # mass, so every pair takes the general form (the headline's equal masses,
+    # as in the reference's initial conditions, take the mass-free form)
+    unequal = None
+    if not args.only:
+        from tests.nbcases import mass_pattern
+        cu = dict(c, pos=mass_pattern(c["pos"], "loguniform"))
+        wall, kms, launches, su = timed_steps(U, U.PREC_MIXED, n, cu, K, W, flags=U.FLAG_NO_NEWTON)
+        unequal = dict(masses="log-uniform in [0.1, 10], total 1", wall_ms=wall, kernel_ms=kms,
+                       launches=launches, tile=su.tile()[1], interactions_per_s=pairs * K / (wall * 1e-3),
+                       kernel_interactions_per_s=pairs * launches / (kms * 1e-3))
+        su.close()
+        log(f"mixed, unequal masses [{unequal['tile']}]: {wall / K:.3f} ms/step wall, "
+            f"{unequal['interactions_per_s']:.4e} interactions/s")
+
     if args.only:
         r = results[args.only]
         print(json.dumps({"metric": METRIC, "value": r["interactions_per_s"], "unit": "interactions/s",
@@ -418,8 +433,10 @@ def main():
             "workload": f"nbody all-pairs force + kick-drift, N={n} Plummer (a=1, M=1), eps=0.002, "
                         f"dt=2.5e-4, {K} steps, FP32 compensated pairs + FP64 state",
             "n": n, "eps": 0.002, "dt": 2.5e-4, "steps": K, "precision": "mixed",
+            "masses": "equal (1/N), as the reference's initial conditions (mass 1 per body)",
             "parallelism": "single GPU",
         },
+        "unequal_masses": unequal,
         "vs_plain_fp32": {
             "fp32_interactions_per_s": results["fp32"]["interactions_per_s"],
             "mixed_over_fp32": m["interactions_per_s"] / results["fp32"]["interactions_per_s"],
