@@ -125,10 +125,20 @@ def cpu_baseline(c, n, threads, rows):
     t0 = time.perf_counter()
     O.gravitation_rows_f32(pos32, eps2, 0, rows, threads)
     dt = time.perf_counter() - t0
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     out = {
         "value": rows * float(n) / dt,
         "unit": "interactions/s",
         "cores": threads,
+        "host_cpus": os.cpu_count(),
+        "cpu_model": model,
         "kind": "port",
         "sample": f"one force evaluation of rows 0..{rows - 1} against all {n} sources "
                   f"(FP32, reference order) = {rows * float(n):.3g} pair interactions, "
@@ -190,6 +200,69 @@ def batched_config5(U, nsys=256, steps=20, seed=0):
     return out
 
 
+def reference_arithmetic(U, c, n, steps=3):
+    """The reference's own arithmetic on the GPU (bit-identical to
+    benchapps/nbody/seq/bodysystem.cc: sqrtf, FP64-rounded divide, sequential
+    j order per body), FP32 state, the headline bodies and softening."""
+    pos = c["pos"].astype(np.float32)
+    vel = c["vel"].astype(np.float32)
+    eps2 = np.float32(c["eps"]) * np.float32(c["eps"])
+    with U.System(n, U.PREC_REFERENCE, float(eps2), float(np.float32(c["dt"])), 1.0,
+                  kick=U.KICK_REFERENCE, state_dtype=U.F32) as s:
+        s.set_state(pos, vel)
+        s.step(1)  # warm-up
+        s.set_state(pos, vel)
+        device_sync(s)
+        t0 = time.perf_counter()
+        s.step(steps)
+        device_sync(s)
+        wall = (time.perf_counter() - t0) * 1e3
+        kms, launches = s.last_kernel_ms()
+    out = dict(steps=steps, ms_per_step=wall / steps, kernel_ms_per_step=kms / launches,
+               interactions_per_s=float(n) * n * steps / (wall * 1e-3),
+               kernel="ref_step_kernel<float, float4, true>: one thread per body, sources in index "
+                      "order, the reference's operation order; bit-identical to its CPU path")
+    log(f"reference arithmetic: {out['ms_per_step']:.2f} ms/step, {out['interactions_per_s']:.4e} interactions/s")
+    return out
+
+
+def config4_fwd_adjoint(U):
+    """N = 16384 uniform cube, log-uniform masses in [0.5, 2]/N, eps = 0.01,
+    dt = 1e-3, 50 steps, checkpoint every 10: the forward alone and forward +
+    adjoint (same loss as fwd_adjoint), mixed precision, default tile."""
+    from tests.nbcases import config4
+    c = config4()
+    n, steps, ckpt = len(c["pos"]), c["steps"], c["ckpt"]
+    pairs = float(n) * n
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+        s.set_state(c["pos"], c["vel"])
+        s.step(steps)  # warm-up
+        s.set_state(c["pos"], c["vel"])
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        fwd = (time.perf_counter() - t0) * 1e3
+        s.set_state(c["pos"], c["vel"])
+        s.adjoint(steps, ckpt, c["target"])  # warm-up (workspace)
+        s.set_state(c["pos"], c["vel"])
+        s.sync()
+        t0 = time.perf_counter()
+        s.adjoint(steps, ckpt, c["target"])
+        adj = (time.perf_counter() - t0) * 1e3
+        _, launches = s.last_kernel_ms()
+        tile = s.tile()[1]
+    out = dict(workload=f"N={n} cube, log-uniform masses, eps=0.01, dt=1e-3, {steps} steps, "
+                        f"checkpoint every {ckpt}", tile=tile,
+               fwd_ms=fwd, fwd_interactions_per_s=pairs * steps / (fwd * 1e-3),
+               fwd_adj_ms=adj, forward_equivalent_interactions_per_s=pairs * steps / (adj * 1e-3),
+               pair_evaluations=launches, ratio_vs_forward=fwd / adj,
+               total_ratio_vs_forward=fwd / adj * launches / steps)
+    log(f"config 4 [{tile}]: forward {fwd:.2f} ms, fwd+adjoint {adj:.2f} ms "
+        f"({out['ratio_vs_forward']:.3f}x forward-equivalent)")
+    return out
+
+
 def fwd_adjoint(U, c, n, K, fwd_rate, ckpt=10):
     """Forward K steps with checkpoints every `ckpt`, then the adjoint of the
     mean squared distance to seeded targets back to the initial state (mixed
@@ -231,7 +304,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=262144,
                     help="rows of the CPU baseline's force evaluation (default: all, ~10 s on 16 threads)")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched"], default=None,
+    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched", "reference"], default=None,
                     help="time one path only and print its line (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 256 MiB device copy (to scale HBM counters) and exit")
@@ -274,6 +347,12 @@ def main():
         r = batched_config5(U)
         print(json.dumps({"metric": METRIC, "value": r["graph_fwd_interactions_per_s"],
                           "unit": "interactions/s", "n_gpus": 1, "only": "batched", "batched": r}),
+              flush=True)
+        return
+    if args.only == "reference":
+        r = reference_arithmetic(U, c, n, K)
+        print(json.dumps({"metric": METRIC, "value": r["interactions_per_s"], "unit": "interactions/s",
+                          "n_gpus": 1, "steps": K, "only": "reference", "reference_arithmetic": r}),
               flush=True)
         return
     if args.only == "adjoint":
@@ -374,6 +453,21 @@ def main():
             fwd_adj = {"error": str(e)}
             log(f"fwd+adjoint failed: {e}")
 
+    ref_arith = None
+    try:
+        ref_arith = reference_arithmetic(U, c, n)
+    except Exception as e:  # reported, never hidden
+        ref_arith = {"error": str(e)}
+        log(f"reference arithmetic failed: {e}")
+
+    cfg4 = None
+    if not args.no_adjoint:
+        try:
+            cfg4 = config4_fwd_adjoint(U)
+        except Exception as e:  # reported, never hidden
+            cfg4 = {"error": str(e)}
+            log(f"config 4 failed: {e}")
+
     # ---- batched systems (config 5) ------------------------------------------------
     batch = None
     if not args.no_batch:
@@ -446,6 +540,8 @@ def main():
         "config2_vs_fp64": cfg2,
         "energy": energy,
         "fwd_adjoint": fwd_adj,
+        "config4_fwd_adjoint": cfg4,
+        "reference_arithmetic": ref_arith,
         "batched": batch,
         "fraction_of_fp32_peak": achieved / PEAK_FP32,
         "roofline": roofline,

@@ -74,6 +74,48 @@ def test_reference_config1_bit_exact(dtype):
     assert np.array_equal(gv, ov)
 
 
+def test_reference_inverse_distance_exhaustive():
+    """The reference kernels' inverse distance for eps2 >= 2^-96 (ref_inv.h):
+    every float d2 in [2^-96, inf] and NaN against the compiler's correctly
+    rounded (float)(1.0 / (double)sqrtf(d2)) on the device, bit for bit."""
+    import json
+    import subprocess
+    exe = Path(U.__file__).resolve().parent / "lib" / "probe_invdist"
+    assert exe.exists(), "build first: python -m utc_amd.build"
+    say("case: probe_invdist (all floats 2^-96..inf)")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    say(f"  {out}")
+    assert r.returncode == 0
+    assert out["mismatches"] == 0
+    assert out["checked"] == 0x7F800001 - 0x0F800000 + 1
+    # the margins the correction steps rely on: v_sqrt_f32 within 1 ulp;
+    # v_rcp_f64 close enough that one Newton step lands within 2^-48
+    assert out["sqrt_f32_max_ulps"] <= 1
+    assert out["rcp_f64_max_rel_err"] ** 2 + 2.0 ** -53 < 2.0 ** -48
+
+
+@pytest.mark.parametrize("eps2", [1e-30, 1e-4])
+def test_reference_f32_far_and_tiny_softening_bit_exact(eps2):
+    """eps2 below 2^-96 takes the generic sqrtf and division; pairs 1e20 apart
+    overflow d2 to inf, where the reference's inverse distance is 0."""
+    rng = np.random.default_rng(3)
+    n = 1000
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    pos[:, 3] = rng.uniform(0.5, 2, n).astype(np.float32)
+    pos[:10, 0] += np.float32(1e20)   # far cluster: d2 = inf against the rest
+    pos[10:20, 1] -= np.float32(3e19)
+    say(f"case: REFERENCE f32 field eps2={eps2} with overflowing pairs")
+    e2 = np.float32(eps2)
+    with U.System(n, U.PREC_REFERENCE, float(e2), 1e-3, 1.0, kick=U.KICK_REFERENCE,
+                  state_dtype=U.F32) as s:
+        s.set_state(pos, np.zeros_like(pos))
+        a = s.accel()
+    f = O.gravitation_f32(pos, e2)
+    assert np.array_equal(a.astype(np.float32), f, equal_nan=True)
+
+
 def test_reference_f32_field_matches_oracle_gravitation():
     n, prm, pos, vel, eps2 = fixture_run_frames(np.float32)
     say("case: REFERENCE f32 field n=1024")

@@ -1,0 +1,95 @@
+// probe_invdist.hip -- exhaustive device check of ref_inv_dist (ref_inv.h).
+//
+// For every float d2 from 2^-96 to FLT_MAX, +inf and a quiet NaN, compares
+// ref_inv_dist(d2) bit for bit with (float)(1.0 / (double)sqrtf(d2)) as the
+// compiler builds it with IEEE sqrt and division (no fast math), and reports
+// the number of mismatches, the first few, and the largest relative error of
+// the raw v_rcp_f64 and v_sqrt_f32 approximations over the same inputs (the
+// margins the two correction steps rely on).  One thread per bit pattern in a
+// grid-stride loop over a 32-bit range; counters via global atomics.
+// Build: python -m utc_amd.build; run: utc_amd/lib/probe_invdist.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "ref_inv.h"
+
+#define CHK(x)                                                                                     \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) {                                                                    \
+            std::printf("HIP error %s at line %d\n", hipGetErrorString(e_), __LINE__);             \
+            return 1;                                                                              \
+        }                                                                                          \
+    } while (0)
+
+struct Result {
+    unsigned long long bad;
+    unsigned long long checked;
+    unsigned int first[8];
+    unsigned int nfirst;
+    unsigned int rcp_err_bits;   // max |v_rcp_f64(s) * s - 1| as float bits
+    unsigned int sqrt_err_ulps;  // max |v_sqrt_f32(x) - sqrtf(x)| in ulps
+};
+
+__device__ __attribute__((noinline)) float reference_inv(float d2)
+{
+    // the reference's expression under IEEE rules (this file has no fast math)
+    return (float)(1.0 / (double)sqrtf(d2));
+}
+
+__global__ void check(uint32_t lo, uint32_t hi, Result* r)
+{
+    unsigned long long bad = 0, n = 0;
+    float rcp_err = 0.f;
+    unsigned int sq_ulps = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint64_t u = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; u < (uint64_t)hi; u += stride) {
+        const float x = __uint_as_float((uint32_t)u);
+        const float want = reference_inv(x);
+        const float got = utcnb::ref_inv_dist(x);
+        ++n;
+        const bool same = __float_as_uint(want) == __float_as_uint(got) || (want != want && got != got);
+        if (!same) {
+            ++bad;
+            const unsigned int k = atomicAdd(&r->nfirst, 1u);
+            if (k < 8)
+                r->first[k] = (uint32_t)u;
+        }
+        if (x <= 3.0e38f) {
+            const float s = sqrtf(x);
+            const double y0 = __builtin_amdgcn_rcp((double)s);
+            rcp_err = fmaxf(rcp_err, (float)fabs(y0 * (double)s - 1.0));
+            const int du = __float_as_int(__builtin_amdgcn_sqrtf(x)) - __float_as_int(s);
+            sq_ulps = max(sq_ulps, (unsigned int)(du < 0 ? -du : du));
+        }
+    }
+    atomicAdd(&r->bad, bad);
+    atomicAdd(&r->checked, n);
+    atomicMax(&r->rcp_err_bits, __float_as_uint(rcp_err));
+    atomicMax(&r->sqrt_err_ulps, sq_ulps);
+}
+
+int main()
+{
+    Result* d = nullptr;
+    CHK(hipMalloc(&d, sizeof(Result)));
+    CHK(hipMemset(d, 0, sizeof(Result)));
+    const uint32_t lo = __builtin_bit_cast(uint32_t, utcnb::kRefInvMin);
+    const uint32_t hi = 0x7F800001u;  // through +inf
+    hipLaunchKernelGGL(check, dim3(8192), dim3(256), 0, 0, lo, hi, d);
+    CHK(hipGetLastError());
+    hipLaunchKernelGGL(check, dim3(1), dim3(64), 0, 0, 0x7FC00000u, 0x7FC00001u, d);  // quiet NaN
+    CHK(hipGetLastError());
+    Result h{};
+    CHK(hipMemcpy(&h, d, sizeof(Result), hipMemcpyDeviceToHost));
+    CHK(hipFree(d));
+    float rcp_err;
+    __builtin_memcpy(&rcp_err, &h.rcp_err_bits, 4);
+    std::printf("{\"checked\": %llu, \"mismatches\": %llu, \"first\": [", h.checked, h.bad);
+    for (unsigned int k = 0; k < h.nfirst && k < 8; ++k)
+        std::printf("%s\"0x%08x\"", k ? ", " : "", h.first[k]);
+    std::printf("], \"rcp_f64_max_rel_err\": %.3e, \"sqrt_f32_max_ulps\": %u}\n", rcp_err, h.sqrt_err_ulps);
+    return h.bad == 0 && h.checked == (unsigned long long)(hi - lo) + 1 ? 0 : 1;
+}

@@ -9,52 +9,77 @@
 // target, so every rounding matches the reference's CPU path.
 //
 // Source bodies are uniform across the wave and arrive through scalar loads.
+// For eps2 >= 2^-96 the FP32 kernels take ref_inv_dist (ref_inv.h): the same
+// correctly rounded sqrtf and division, without the generic rescaling.
 #include "internal.h"
+#include "ref_inv.h"
 
 namespace utcnb {
 
 #pragma clang fp contract(off)
 
-template <typename T>
+template <typename T, bool FAST>
 __device__ __forceinline__ T inv_dist(T d2);
 
 template <>
-__device__ __forceinline__ float inv_dist<float>(float d2)
+__device__ __forceinline__ float inv_dist<float, false>(float d2)
 {
     // (float)(1.0 / (double)sqrtf(d2)) -- the literal 1.0 makes the division FP64.
     return (float)(1.0 / (double)sqrtf(d2));
 }
 
 template <>
-__device__ __forceinline__ double inv_dist<double>(double d2)
+__device__ __forceinline__ float inv_dist<float, true>(float d2)
+{
+    return ref_inv_dist(d2);  // d2 >= eps2 >= kRefInvMin
+}
+
+template <>
+__device__ __forceinline__ double inv_dist<double, false>(double d2)
 {
     return 1.0 / sqrt(d2);
 }
 
-template <typename T, typename T4>
+// One source j in the reference's order of operations.
+template <typename T, typename T4, bool FAST>
+__device__ __forceinline__ void ref_pair(const T4& pj, const T4& pi, T eps2, T& a0, T& a1, T& a2)
+{
+    T r0 = pj.x - pi.x;
+    T r1 = pj.y - pi.y;
+    T r2 = pj.z - pi.z;
+    T d2 = r0 * r0 + r1 * r1 + r2 * r2;
+    d2 += eps2;
+    T inv = inv_dist<T, FAST>(d2);
+    T inv3 = inv * inv * inv;
+    T s = pj.w * inv3;
+    a0 += r0 * s;
+    a1 += r1 * s;
+    a2 += r2 * s;
+}
+
+// Sources in index order.  Four per trip: their loads and inverse distances
+// are independent, so they overlap; the sums still take j = 0, 1, 2, ... in
+// turn (no reassociation in this file).
+template <typename T, typename T4, bool FAST>
 __device__ __forceinline__ void ref_field(const T4* __restrict__ pos, int n, T eps2, const T4& pi,
                                           T& a0, T& a1, T& a2)
 {
     a0 = 0; a1 = 0; a2 = 0;
-    for (int j = 0; j < n; ++j) {
-        const T4 pj = pos[j];
-        T r0 = pj.x - pi.x;
-        T r1 = pj.y - pi.y;
-        T r2 = pj.z - pi.z;
-        T d2 = r0 * r0 + r1 * r1 + r2 * r2;
-        d2 += eps2;
-        T inv = inv_dist<T>(d2);
-        T inv3 = inv * inv * inv;
-        T s = pj.w * inv3;
-        a0 += r0 * s;
-        a1 += r1 * s;
-        a2 += r2 * s;
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+        const T4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+        ref_pair<T, T4, FAST>(q0, pi, eps2, a0, a1, a2);
+        ref_pair<T, T4, FAST>(q1, pi, eps2, a0, a1, a2);
+        ref_pair<T, T4, FAST>(q2, pi, eps2, a0, a1, a2);
+        ref_pair<T, T4, FAST>(q3, pi, eps2, a0, a1, a2);
     }
+    for (; j < n; ++j)
+        ref_pair<T, T4, FAST>(pos[j], pi, eps2, a0, a1, a2);
 }
 
 // Fused force + integrate; the twin of the reference's
 // _integrateNBodySystem_kernel<T> (benchapps/nbody/cuda/bodysystem_kernel.cu:87-138).
-template <typename T, typename T4>
+template <typename T, typename T4, bool FAST>
 __global__ void ref_step_kernel(const T4* __restrict__ pos_old, T4* __restrict__ pos_new,
                                 T4* __restrict__ vel, int n, T dt, T eps2, T damping)
 {
@@ -63,7 +88,7 @@ __global__ void ref_step_kernel(const T4* __restrict__ pos_old, T4* __restrict__
         return;
     T4 p = pos_old[i];
     T a0, a1, a2;
-    ref_field<T, T4>(pos_old, n, eps2, p, a0, a1, a2);
+    ref_field<T, T4, FAST>(pos_old, n, eps2, p, a0, a1, a2);
     T w = p.w;  // the reference's "invMass" (holds m_i), seq/bodysystem.cc:149
     T4 v = vel[i];
     v.x += (a0 * w) * dt;
@@ -79,7 +104,7 @@ __global__ void ref_step_kernel(const T4* __restrict__ pos_old, T4* __restrict__
     vel[i] = v;
 }
 
-template <typename T, typename T4>
+template <typename T, typename T4, bool FAST>
 __global__ void ref_accel_kernel(const T4* __restrict__ pos, int n, T eps2,
                                  double* __restrict__ acc3)
 {
@@ -87,7 +112,7 @@ __global__ void ref_accel_kernel(const T4* __restrict__ pos, int n, T eps2,
     if (i >= n)
         return;
     T a0, a1, a2;
-    ref_field<T, T4>(pos, n, eps2, pos[i], a0, a1, a2);
+    ref_field<T, T4, FAST>(pos, n, eps2, pos[i], a0, a1, a2);
     acc3[3 * (size_t)i + 0] = (double)a0;
     acc3[3 * (size_t)i + 1] = (double)a1;
     acc3[3 * (size_t)i + 2] = (double)a2;
@@ -98,8 +123,12 @@ hipError_t launch_ref_step_f32(const float4* pos_old, float4* pos_new, float4* v
 {
     if (block <= 0)
         block = 256;
-    hipLaunchKernelGGL((ref_step_kernel<float, float4>), dim3((n + block - 1) / block),
-                       dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
+    if (eps2 >= kRefInvMin && eps2 <= 3.0e38f)
+        hipLaunchKernelGGL((ref_step_kernel<float, float4, true>), dim3((n + block - 1) / block),
+                           dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
+    else
+        hipLaunchKernelGGL((ref_step_kernel<float, float4, false>), dim3((n + block - 1) / block),
+                           dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
     return hipGetLastError();
 }
 
@@ -109,7 +138,7 @@ hipError_t launch_ref_step_f64(const double4* pos_old, double4* pos_new, double4
 {
     if (block <= 0)
         block = 256;
-    hipLaunchKernelGGL((ref_step_kernel<double, double4>), dim3((n + block - 1) / block),
+    hipLaunchKernelGGL((ref_step_kernel<double, double4, false>), dim3((n + block - 1) / block),
                        dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
     return hipGetLastError();
 }
@@ -117,15 +146,19 @@ hipError_t launch_ref_step_f64(const double4* pos_old, double4* pos_new, double4
 hipError_t launch_ref_accel_f32(const float4* src, int n, float eps2, double* acc3,
                                 hipStream_t st)
 {
-    hipLaunchKernelGGL((ref_accel_kernel<float, float4>), dim3((n + 255) / 256), dim3(256), 0,
-                       st, src, n, eps2, acc3);
+    if (eps2 >= kRefInvMin && eps2 <= 3.0e38f)
+        hipLaunchKernelGGL((ref_accel_kernel<float, float4, true>), dim3((n + 255) / 256), dim3(256), 0,
+                           st, src, n, eps2, acc3);
+    else
+        hipLaunchKernelGGL((ref_accel_kernel<float, float4, false>), dim3((n + 255) / 256), dim3(256), 0,
+                           st, src, n, eps2, acc3);
     return hipGetLastError();
 }
 
 hipError_t launch_ref_accel_f64(const double4* src, int n, double eps2, double* acc3,
                                 hipStream_t st)
 {
-    hipLaunchKernelGGL((ref_accel_kernel<double, double4>), dim3((n + 255) / 256), dim3(256),
+    hipLaunchKernelGGL((ref_accel_kernel<double, double4, false>), dim3((n + 255) / 256), dim3(256),
                        0, st, src, n, eps2, acc3);
     return hipGetLastError();
 }

@@ -1,0 +1,62 @@
+// ref_inv.h -- the reference's inverse distance (float)(1.0 / (double)sqrtf(d2))
+// (benchapps/nbody/seq/bodysystem.h:86) in fewer instructions, bit for bit,
+// for d2 >= 2^-96 (device code only).
+//
+// sqrtf.  v_sqrt_f32 returns s0 within 1 ulp of sqrt(x), so the correctly
+// rounded root is s0 or a neighbour.  Tuckerman's test picks it: sqrt(x) <
+// s0 - u/2 (u = ulp of s0) iff x < (s0 - u/2)^2 = s0 (s0 - u) + u^2/4.  The
+// square of a float midpoint is never a float and x lies on a grid far
+// coarser than u^2/4, so x <= s0 (s0 - u) decides, and one FMA gives the sign
+// of x - s0 (s0 - u) exactly; likewise x > s0 (s0 + u) for the upper
+// neighbour.  (s0 - u and s0 + u are the neighbouring floats: the bit pattern
+// minus and plus one.)
+//
+// 1.0 / (double)s, rounded to float.  Double rounding is innocuous here, and
+// more: 1/s stays at least 2^-48 (relative) away from every float midpoint
+// (|1 - s m| >= 2^-48 for a float s and a midpoint m: s m is a 24-by-25-bit
+// product that cannot equal 1), so ANY FP64 approximation of 1/s within
+// 2^-49 rounds to the same float.  v_rcp_f64 and one Newton-Raphson step
+// y1 = y0 + y0 (1 - s y0) give that.  The clamp of s in the residual keeps
+// 1/inf = 0 (an overflowed d2) instead of inf * 0; NaN stays NaN.
+//
+// The compiler's generic sequences rescale tiny inputs; from 2^-96 up they
+// are not needed (s and 1/s are normal).  utc_amd/bench/probe_invdist.hip
+// checks this function against the compiler's correctly rounded sqrtf and
+// division for every float d2 in [2^-96, inf] and NaN on the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace utcnb {
+
+// Smallest d2 for which ref_inv_dist is exact.
+constexpr float kRefInvMin = 0x1p-96f;
+
+__device__ __forceinline__ float ref_sqrt_rn(float x)
+{
+    const float s0 = __builtin_amdgcn_sqrtf(x);
+    const float lo = __int_as_float(__float_as_int(s0) - 1);
+    const float hi = __int_as_float(__float_as_int(s0) + 1);
+    float s = s0;
+    if (__builtin_fmaf(-lo, s0, x) <= 0.f)
+        s = lo;
+    if (__builtin_fmaf(-hi, s0, x) > 0.f)
+        s = hi;
+    return s;
+}
+
+__device__ __forceinline__ float ref_recip_rn(float s)
+{
+    const double d = (double)s;
+    const double y0 = __builtin_amdgcn_rcp(d);
+    const double e = __builtin_fma(-__builtin_fmin(d, 0x1p1000), y0, 1.0);
+    return (float)__builtin_fma(y0, e, y0);
+}
+
+// (float)(1.0 / (double)sqrtf(d2)) for d2 >= kRefInvMin (or +inf, NaN).
+__device__ __forceinline__ float ref_inv_dist(float d2)
+{
+    return ref_recip_rn(ref_sqrt_rn(d2));
+}
+
+}  // namespace utcnb
