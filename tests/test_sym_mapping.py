@@ -55,3 +55,41 @@ def test_every_ordered_pair_of_blocks_once(R, K):
         assert all(c == 1 for c in hits.values()), (NW, R, K)
         assert all(c == 1 for c in writes.values()), (NW, R, K)
         assert reads == writes, (NW, R, K)
+
+
+def reduce_lanes(bodies):
+    """sym_reduce_lanes (utc_amd/csrc/sym_reduce.h)."""
+    T = 1
+    while T < 16 and bodies * T < (1 << 19):
+        T *= 2
+    return T
+
+
+@pytest.mark.parametrize("T", [1, 2, 4, 8, 16])
+def test_reduction_lanes_cover_every_body(T):
+    """sym_reduce_lane<T> over the launch grid ceil(n T / 256) x 256: every body
+    below n gets T lanes t = 0..T-1 of one wave, the butterfly offsets
+    32, 16, .., 64/T flip exactly the bits of t, and clamped lanes read body
+    n - 1 (a valid record) while writing nothing."""
+    for n in (1, 5, 63, 64, 65, 1000, 4097):
+        blocks = (n * T + 255) // 256
+        seen = {}
+        for g in range(blocks * 256):
+            lane = g & 63
+            t, j = divmod(lane, 64 // T)
+            body = (g >> 6) * (64 // T) + j
+            read = min(body, n - 1)
+            assert 0 <= read < n
+            if body < n:
+                seen.setdefault(body, []).append((g >> 6, t))
+        assert sorted(seen) == list(range(n))
+        for body, lanes in seen.items():
+            assert sorted(t for _, t in lanes) == list(range(T))
+            assert len({wave for wave, _ in lanes}) == 1
+        offs = []
+        off = 32
+        while off >= 64 // T:
+            offs.append(off)
+            off //= 2
+        assert sum(offs) == 64 - 64 // T  # together they span the t bits
+    assert reduce_lanes(262144) == 2 and reduce_lanes(16384) == 16 and reduce_lanes(1 << 19) == 1

@@ -28,6 +28,7 @@
 // partials go to HBM, and a second kernel adds them per body in a fixed order
 // (deterministic) and applies the integrator epilogue.
 #include "force_tmpl.h"
+#include "sym_reduce.h"
 
 namespace utcnb {
 
@@ -234,73 +235,45 @@ __global__ void batch_counter_reset_kernel(int* counter)
         *counter = 0;
 }
 
-// Sum of the partials of body b (index within its system) in FP64, in a
-// fixed order: the R target-side partials, then the reactions by distance d.
-__device__ __forceinline__ void sym_sum(const float* __restrict__ ipart, const float* __restrict__ jpart,
-                                        int b, int NW, int R, int K, double s[3])
-{
-    const int n_pad = NW * 64 * K;
-    const int half = NW / 2, v = b / (64 * K), loc = b - v * (64 * K);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int r = 0; r < R; ++r) {
-        const float* p = ipart + ((size_t)r * n_pad + b) * 3;
-        s0 += p[0];
-        s1 += p[1];
-        s2 += p[2];
-    }
-    for (int d = 0; d < half; ++d) {
-        int w = v - 1 - d;
-        if (w < 0)
-            w += NW;
-        if (2 * (d + 1) == NW && w >= half)
-            continue;
-        const float* p = jpart + (((size_t)w * half + d) * (64 * K) + loc) * 3;
-        s0 += p[0];
-        s1 += p[1];
-        s2 += p[2];
-    }
-    s[0] = s0;
-    s[1] = s1;
-    s[2] = s2;
-}
-
 // The epilogues take (hi, c) with value hi - c; split an FP64 sum exactly.
 template <class Epi>
-__device__ __forceinline__ void sym_epilogue(const Epi& epi, int i, const double s[3])
+__device__ __forceinline__ void sym_epilogue(const Epi& epi, int i, const double (&s)[3])
 {
     const float h0 = (float)s[0], h1 = (float)s[1], h2 = (float)s[2];
     epi(i, h0, h1, h2, (float)((double)h0 - s[0]), (float)((double)h1 - s[1]), (float)((double)h2 - s[2]));
 }
 
-template <class Epi>
+// Body b's partials summed in FP64 by T lanes (sym_reduce.h), then the epilogue.
+template <int T, class Epi>
 __global__ __launch_bounds__(256) void sym_reduce_kernel(const float* __restrict__ ipart,
                                                          const float* __restrict__ jpart, int n, int NW,
                                                          int R, int K, Epi epi)
 {
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= n)
-        return;
+    int b, t;
+    sym_reduce_lane<T>(b, t);
     double s[3];
-    sym_sum(ipart, jpart, b, NW, R, K, s);
-    sym_epilogue(epi, b, s);
+    sym_split_sum<T>((const Part3*)ipart, (const Part3*)jpart, b < n ? b : n - 1, NW, R, K, t, s);
+    if (b < n && t == 0)
+        sym_epilogue(epi, b, s);
 }
 
 // Batched systems: body b of the concatenated padded layout (systems start on
 // multiples of kBatchSymAlign, so block b / kBatchSymAlign names the system).
+template <int T>
 __global__ __launch_bounds__(256) void batch_sym_reduce_kernel(BatchSymGeom g, int total, int K,
                                                                const float* __restrict__ ipart,
                                                                const float* __restrict__ jpart, EpiMixed epi)
 {
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= total)
-        return;
-    const int s = g.sys_of_block[b / kBatchSymAlign];
-    const int loc = b - g.off[s];
-    if (loc >= g.n[s])
-        return;  // padding bodies keep their far-away zero-mass records
+    int b, t;
+    sym_reduce_lane<T>(b, t);
+    const int bb = b < total ? b : total - 1;
+    const int s = g.sys_of_block[bb / kBatchSymAlign];
+    const int loc = bb - g.off[s];  // padding bodies are summed too (their records exist)
     double sum[3];
-    sym_sum(ipart + g.ipo[s], jpart + g.jpo[s], loc, g.nw[s], g.rs[s], K, sum);
-    sym_epilogue(epi, b, sum);
+    sym_split_sum<T>((const Part3*)(ipart + g.ipo[s]), (const Part3*)(jpart + g.jpo[s]), loc, g.nw[s], g.rs[s],
+                     K, t, sum);
+    if (b < total && loc < g.n[s] && t == 0)
+        sym_epilogue(epi, b, sum);  // padding bodies keep their far-away zero-mass records
 }
 
 size_t sym_ipart_floats(int n_pad, int K, int R) { (void)K; return (size_t)R * n_pad * 3; }
@@ -341,8 +314,15 @@ hipError_t launch_sym(const Tile& t, bool newton, const float4* src, int n, int 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL((sym_reduce_kernel<Epi>), dim3((n + 255) / 256), dim3(256), 0, st, ipart, jpart, n,
-                       NW, R, K, epi);
+    const int T = sym_reduce_lanes(n);
+    const dim3 rgrid((unsigned)(((long long)n * T + 255) / 256));
+    switch (T) {
+    case 1: hipLaunchKernelGGL((sym_reduce_kernel<1, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    case 2: hipLaunchKernelGGL((sym_reduce_kernel<2, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    case 4: hipLaunchKernelGGL((sym_reduce_kernel<4, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    case 8: hipLaunchKernelGGL((sym_reduce_kernel<8, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    default: hipLaunchKernelGGL((sym_reduce_kernel<16, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    }
     return hipGetLastError();
 }
 
@@ -365,8 +345,15 @@ hipError_t launch_batch_sym_step(const BatchSymGeom& g, int nwaves, int total, c
     if (e != hipSuccess)
         return e;
     EpiMixed epi{pos, vel, snap_new, dt, damping, kick};
-    hipLaunchKernelGGL(batch_sym_reduce_kernel, dim3((total + 255) / 256), dim3(256), 0, st, g, total,
-                       kBatchSymK, ipart, jpart, epi);
+    const int T = sym_reduce_lanes(total);
+    const dim3 rgrid((unsigned)(((long long)total * T + 255) / 256));
+    switch (T) {
+    case 1: hipLaunchKernelGGL(batch_sym_reduce_kernel<1>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, epi); break;
+    case 2: hipLaunchKernelGGL(batch_sym_reduce_kernel<2>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, epi); break;
+    case 4: hipLaunchKernelGGL(batch_sym_reduce_kernel<4>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, epi); break;
+    case 8: hipLaunchKernelGGL(batch_sym_reduce_kernel<8>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, epi); break;
+    default: hipLaunchKernelGGL(batch_sym_reduce_kernel<16>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, epi); break;
+    }
     return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@
 // Partials go to HBM (float4 per body: x, y, z, m) and a second kernel adds
 // them per body in FP64, in a fixed order, into xbar and mbar.
 #include "internal.h"
+#include "sym_reduce.h"
 #include "vjp_pair.h"
 
 namespace utcnb {
@@ -260,70 +261,48 @@ __global__ __launch_bounds__(256) void batch_sym_vjp_kernel(BatchSymGeom g, int 
     }
 }
 
-// Sum of body b's partials (index within its system), FP64, fixed order.
-__device__ __forceinline__ void sym_vjp_sum(const float4* __restrict__ ipart, const float4* __restrict__ jpart,
-                                            int b, int NW, int R, int K, double s[4])
-{
-    const int n_pad = NW * 64 * K;
-    const int half = NW / 2, v = b / (64 * K), loc = b - v * (64 * K);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    for (int r = 0; r < R; ++r) {
-        const float4 p = ipart[(size_t)r * n_pad + b];
-        s0 += p.x; s1 += p.y; s2 += p.z; s3 += p.w;
-    }
-    for (int d = 0; d < half; ++d) {
-        int w = v - 1 - d;
-        if (w < 0)
-            w += NW;
-        if (2 * (d + 1) == NW && w >= half)
-            continue;
-        const float4 p = jpart[((size_t)w * half + d) * (64 * K) + loc];
-        s0 += p.x; s1 += p.y; s2 += p.z; s3 += p.w;
-    }
-    s[0] = s0;
-    s[1] = s1;
-    s[2] = s2;
-    s[3] = s3;
-}
-
-// xbar_b += sum of body b's partials, mbar_b likewise.
+// xbar_b += the sum of body b's partials, mbar_b likewise (T lanes per body,
+// sym_reduce.h).
+template <int T>
 __global__ __launch_bounds__(256) void sym_vjp_reduce_kernel(const float4* __restrict__ ipart,
                                                              const float4* __restrict__ jpart, int n,
                                                              int NW, int R, int K,
                                                              double* __restrict__ xbar3,
                                                              double* __restrict__ mbar)
 {
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= n)
-        return;
+    int b, t;
+    sym_reduce_lane<T>(b, t);
     double s[4];
-    sym_vjp_sum(ipart, jpart, b, NW, R, K, s);
-    xbar3[3 * (size_t)b + 0] += s[0];
-    xbar3[3 * (size_t)b + 1] += s[1];
-    xbar3[3 * (size_t)b + 2] += s[2];
-    mbar[b] += s[3];
+    sym_split_sum<T>(ipart, jpart, b < n ? b : n - 1, NW, R, K, t, s);
+    if (b < n && t == 0) {
+        xbar3[3 * (size_t)b + 0] += s[0];
+        xbar3[3 * (size_t)b + 1] += s[1];
+        xbar3[3 * (size_t)b + 2] += s[2];
+        mbar[b] += s[3];
+    }
 }
 
 // Batched systems: body b of the concatenated padded layout.
+template <int T>
 __global__ __launch_bounds__(256) void batch_sym_vjp_reduce_kernel(BatchSymGeom g, int total, int K,
                                                                    const float4* __restrict__ ipart,
                                                                    const float4* __restrict__ jpart,
                                                                    double* __restrict__ xbar3,
                                                                    double* __restrict__ mbar)
 {
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= total)
-        return;
-    const int s = g.sys_of_block[b / kBatchSymAlign];
-    const int loc = b - g.off[s];
-    if (loc >= g.n[s])
-        return;
+    int b, t;
+    sym_reduce_lane<T>(b, t);
+    const int bb = b < total ? b : total - 1;
+    const int s = g.sys_of_block[bb / kBatchSymAlign];
+    const int loc = bb - g.off[s];
     double sum[4];
-    sym_vjp_sum(ipart + g.vipo[s], jpart + g.vjpo[s], loc, g.nw[s], g.rs[s], K, sum);
-    xbar3[3 * (size_t)b + 0] += sum[0];
-    xbar3[3 * (size_t)b + 1] += sum[1];
-    xbar3[3 * (size_t)b + 2] += sum[2];
-    mbar[b] += sum[3];
+    sym_split_sum<T>(ipart + g.vipo[s], jpart + g.vjpo[s], loc, g.nw[s], g.rs[s], K, t, sum);
+    if (b < total && loc < g.n[s] && t == 0) {
+        xbar3[3 * (size_t)b + 0] += sum[0];
+        xbar3[3 * (size_t)b + 1] += sum[1];
+        xbar3[3 * (size_t)b + 2] += sum[2];
+        mbar[b] += sum[3];
+    }
 }
 
 constexpr int kSymVjpK = 4;
@@ -362,8 +341,15 @@ hipError_t launch_sym_vjp(const float4* jrec, int n, int n_pad, float eps2, int 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(sym_vjp_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ipart, jpart, n,
-                       NW, R, kSymVjpK, xbar3, mbar);
+    const int T = sym_reduce_lanes(n);
+    const dim3 rgrid((unsigned)(((long long)n * T + 255) / 256));
+    switch (T) {
+    case 1: hipLaunchKernelGGL(sym_vjp_reduce_kernel<1>, rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, kSymVjpK, xbar3, mbar); break;
+    case 2: hipLaunchKernelGGL(sym_vjp_reduce_kernel<2>, rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, kSymVjpK, xbar3, mbar); break;
+    case 4: hipLaunchKernelGGL(sym_vjp_reduce_kernel<4>, rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, kSymVjpK, xbar3, mbar); break;
+    case 8: hipLaunchKernelGGL(sym_vjp_reduce_kernel<8>, rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, kSymVjpK, xbar3, mbar); break;
+    default: hipLaunchKernelGGL(sym_vjp_reduce_kernel<16>, rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, kSymVjpK, xbar3, mbar); break;
+    }
     return hipGetLastError();
 }
 
@@ -380,8 +366,15 @@ hipError_t launch_batch_sym_vjp(const BatchSymGeom& g, int nwaves, int total, co
     e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(batch_sym_vjp_reduce_kernel, dim3((total + 255) / 256), dim3(256), 0, st, g, total,
-                       kBatchSymK, ipart, jpart, xbar3, mbar);
+    const int T = sym_reduce_lanes(total);
+    const dim3 rgrid((unsigned)(((long long)total * T + 255) / 256));
+    switch (T) {
+    case 1: hipLaunchKernelGGL(batch_sym_vjp_reduce_kernel<1>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, xbar3, mbar); break;
+    case 2: hipLaunchKernelGGL(batch_sym_vjp_reduce_kernel<2>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, xbar3, mbar); break;
+    case 4: hipLaunchKernelGGL(batch_sym_vjp_reduce_kernel<4>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, xbar3, mbar); break;
+    case 8: hipLaunchKernelGGL(batch_sym_vjp_reduce_kernel<8>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, xbar3, mbar); break;
+    default: hipLaunchKernelGGL(batch_sym_vjp_reduce_kernel<16>, rgrid, dim3(256), 0, st, g, total, kBatchSymK, ipart, jpart, xbar3, mbar); break;
+    }
     return hipGetLastError();
 }
 
