@@ -297,6 +297,26 @@ def test_full_size_accuracy_against_fp64(which):
     assert errs[U.PREC_FP32] <= 1e-6
 
 
+def test_large_n_symmetric_default_matches_fp64():
+    """N = 1.5M: the symmetric kernel's partial sums (26 GB) fit the device's
+    budget (a quarter of its memory, at most 64 GiB), so the default tile stays
+    symmetric; its field against the on-device FP64 path."""
+    n = 1_500_000
+    c = plummer(n, eps=0.002, dt=2.5e-4)
+    say(f"case: large N={n} default tile vs FP64 field")
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"]) as s:
+        name = s.tile()[1]
+        assert name.startswith("sym"), name
+        s.set_state(c["pos"], c["vel"])
+        a = s.accel()
+    with U.System(n, U.PREC_FP64, c["eps"] ** 2, c["dt"]) as s64:
+        s64.set_state(c["pos"], c["vel"])
+        a64 = s64.accel()
+    err = relnorm(a, a64)
+    say(f"  tile {name}: accel rel err {err:.3e}")
+    assert err <= 2e-6
+
+
 def test_autotune_picks_a_valid_tile_and_caches_it_by_name(tmp_path):
     """utcnb_autotune times every tile for the size class (N rounded up to 1024),
     returns a tile usable in utcnb_config.tile, and keeps the result in memory

@@ -64,6 +64,7 @@ struct utcnb_batch {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int cu_count = 256;
+    size_t partial_budget = (size_t)16 << 30;  // sym_partial_budget of the device
     int nsys = 0;
     std::vector<int> n, npad, off;
     long long total = 0, total_real = 0;
@@ -303,19 +304,16 @@ hipError_t upload(T** d, const std::vector<T>& h)
     return hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
 }
 
-// Partial sums of the symmetric kernels are kept under this many bytes (force
-// and VJP each); batches above it use the one-pass kernels.
-constexpr size_t kSymPartialLimit = (size_t)16 << 30;
-
 // Sets up the symmetric kernels; returns hipSuccess with b->sym_ok false when
-// their partial sums would exceed kSymPartialLimit.
+// their partial sums (force or VJP) would exceed the device's
+// sym_partial_budget; such batches use the one-pass kernels.
 hipError_t setup_sym(utcnb_batch* b)
 {
     SymPlan P;
     plan_sym(b, b->cu_count * 16, P);
     const size_t fbytes = (size_t)(P.ifloats + P.jfloats) * sizeof(float);
     const size_t vbytes = (size_t)(P.vi4 + P.vj4) * sizeof(float4);
-    if (fbytes > kSymPartialLimit || vbytes > kSymPartialLimit)
+    if (fbytes > b->partial_budget || vbytes > b->partial_budget)
         return hipSuccess;
     hipError_t e = upload(&b->d_nw, P.nw);
     if (e == hipSuccess) e = upload(&b->d_rs, P.rs);
@@ -542,6 +540,7 @@ int utcnb_batch_create(int nsys, const int32_t* sizes, const utcnb_config* cfg, 
         return fail(UTCNB_ENOMEM, "utcnb_batch_create: out of host memory");
     b->cfg = *cfg;
     b->cu_count = prop.multiProcessorCount;
+    b->partial_budget = sym_partial_budget(prop.totalGlobalMem);
     b->nsys = nsys;
     // the symmetric kernels (candidates when tuning) need systems in whole wave blocks
     b->align = (cfg->tile == 0 || cfg->tile > kSymPersist) ? kBatchSymAlign : kChunk;

@@ -76,11 +76,12 @@ struct utcnb_system {
     } adj;
     bool timed = false;
     int cu_count = 256;
+    size_t partial_budget = (size_t)16 << 30;  // sym_partial_budget of the device
 };
 
 namespace {
 
-int check_device(int* dev, int* cus)
+int check_device(int* dev, int* cus, size_t* mem = nullptr)
 {
     int d = 0;
     if (hipGetDevice(&d) != hipSuccess)
@@ -94,6 +95,8 @@ int check_device(int* dev, int* cus)
     *dev = d;
     if (cus)
         *cus = prop.multiProcessorCount;
+    if (mem)
+        *mem = prop.totalGlobalMem;
     return UTCNB_OK;
 }
 
@@ -325,7 +328,8 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
         return fail(UTCNB_EINVAL, "utcnb_create: tile out of range");
 
     int dev = 0, cus = 256;
-    int rc = check_device(&dev, &cus);
+    size_t mem = 0;
+    int rc = check_device(&dev, &cus, &mem);
     if (rc)
         return rc;
 
@@ -336,6 +340,7 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
     s->stream = (hipStream_t)stream;
     s->device = dev;
     s->cu_count = cus;
+    s->partial_budget = sym_partial_budget(mem);
     s->n = c.n;
     const bool fast = c.precision != UTCNB_PREC_REFERENCE;
     s->n_pad = fast ? (int)pad_j(c.n) : c.n;
@@ -349,7 +354,7 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
     s->args.damping = c.damping;
     s->args.kick = c.kick;
     s->args.newton = (c.flags & UTCNB_FLAG_NO_NEWTON) == 0;
-    s->args.tile = c.tile > 0 ? c.tile - 1 : tile_default(c.n, cus);
+    s->args.tile = c.tile > 0 ? c.tile - 1 : tile_default(c.n, cus, s->partial_budget);
 
     switch (c.precision) {
     case UTCNB_PREC_REFERENCE:
@@ -376,12 +381,12 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
         if (np % (size_t)(128 * t.k) != 0) {
             rc = fail(UTCNB_EINVAL, "utcnb_create: symmetric tile needs N padded to 128*K");
         } else {
-            const size_t jf = sym_jpart_floats((int)np, t.k);
-            if (jf * sizeof(float) > ((size_t)16 << 30))
-                rc = fail(UTCNB_EINVAL, "utcnb_create: N too large for a symmetric tile (reaction "
-                                        "partials above 16 GiB); use a one-pass tile");
+            const size_t jf = sym_jpart_floats((int)np, t.k), jif = sym_ipart_floats((int)np, t.k, t.s);
+            if ((jf + jif) * sizeof(float) > s->partial_budget)
+                rc = fail(UTCNB_EINVAL, "utcnb_create: N too large for a symmetric tile (partial sums "
+                                        "above a quarter of the device, at most 64 GiB); use a one-pass tile");
             else
-                rc = dalloc(&s->ipart, sym_ipart_floats((int)np, t.k, t.s)) || dalloc(&s->jpart, jf);
+                rc = dalloc(&s->ipart, jif) || dalloc(&s->jpart, jf);
         }
         s->args.ipart = s->ipart;
         s->args.jpart = s->jpart;
@@ -632,10 +637,12 @@ static int adjoint_workspace(utcnb_system* s, int nck, int seg)
              dalloc(&a.tgt, 3 * n) || dalloc(&a.part, (n + 255) / 256 + 1);
     if (!rc && f32)
         rc = dalloc(&a.wsnap[0], np) || dalloc(&a.wsnap[1], np);
+    const int Rv = sym_vjp_splits((int)np, s->cu_count);
     if (!rc && f32 && tile_get(s->args.tile).sym && sym_vjp_applies((int)np) &&
-        sym_vjp_jpart_float4s((int)np) * sizeof(float4) <= ((size_t)16 << 30)) {
+        (sym_vjp_jpart_float4s((int)np) + sym_vjp_ipart_float4s((int)np, Rv)) * sizeof(float4) <=
+            s->partial_budget) {
         // a symmetric force tile also evaluates the adjoint pair sum symmetrically
-        const int R = sym_vjp_splits((int)np, s->cu_count);
+        const int R = Rv;
         rc = dalloc(&a.vjp_ipart, sym_vjp_ipart_float4s((int)np, R)) ||
              dalloc(&a.vjp_jpart, sym_vjp_jpart_float4s((int)np));
         a.vjp_r = R;

@@ -1,5 +1,7 @@
 // force_fp32.hip -- launchers of the FP32 pair kernels (plain and mixed steps,
 // field-only evaluation).  The kernel itself is in force_tmpl.h.
+#include <algorithm>
+
 #include "force_tmpl.h"
 
 namespace utcnb {
@@ -37,13 +39,21 @@ const Tile& tile_get(int id) { return kTiles[id]; }
 // The symmetric tiles need enough wave blocks to fill the chip (K = 8 from 128
 // bodies per CU, K = 4 from 32), and their reaction partials must fit the
 // 16 GiB workspace limit; above it the best one-pass tile (k2s8) is used.
-int tile_default(int64_t n, int cu_count)
+size_t sym_partial_budget(size_t device_bytes)
+{
+    if (device_bytes == 0)
+        return (size_t)16 << 30;
+    return std::min<size_t>((size_t)64 << 30, device_bytes / 4);
+}
+
+int tile_default(int64_t n, int cu_count, size_t partial_budget)
 {
     const int64_t cus = cu_count > 0 ? cu_count : 256;
     const int64_t n_pad = pad_j(n);
     const auto fits = [&](int k) {
         return n_pad <= (1 << 30) && n_pad % (128 * k) == 0 &&
-               sym_jpart_floats((int)n_pad, k) * sizeof(float) <= ((size_t)16 << 30);
+               (sym_jpart_floats((int)n_pad, k) + sym_ipart_floats((int)n_pad, k, 32)) * sizeof(float) <=
+                   partial_budget;
     };
     if (n >= 128 * cus && fits(8))
         return 17;  // sym8r32

@@ -42,7 +42,11 @@ struct Tile {
 int tile_count();
 const Tile& tile_get(int id);
 // Heuristic default: enough waves to cover the chip (256 CUs x 8 waves).
-int tile_default(int64_t n, int cu_count);
+// Device memory the symmetric kernels may take for their partial sums (force
+// and VJP each): a quarter of the device, at most 64 GiB (16 GiB when the
+// device size is unknown).  Larger systems use one-pass kernels.
+size_t sym_partial_budget(size_t device_bytes);
+int tile_default(int64_t n, int cu_count, size_t partial_budget);
 
 // ---- launchers (force.hip) ------------------------------------------------
 struct StepArgs {
