@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import utc_amd as U
-from nbcases import config5_sizes, relnorm
+from nbcases import config5_sizes, mass_pattern, relnorm
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -119,6 +119,37 @@ def test_batched_adjoint_matches_single_system(tile):
             assert relnorm(mine[0][sl], gx) <= 1e-5
             assert relnorm(mine[1][sl], gv) <= 1e-5
             assert relnorm(mine[2][sl], gm) <= 1e-5
+
+
+@pytest.mark.parametrize("tile", [4, 5])
+def test_batched_symmetric_mass_patterns(tile):
+    """Systems with one mass, a few heavy bodies and log-uniform masses in one
+    batch: the symmetric kernels' mass-free and general pair forms side by
+    side, forward against the FP64 oracle and adjoint against the FP64 system."""
+    eps2, dt, steps, ck = 1e-4, 1e-3, 4, 2
+    sizes = [1000, 2048, 777, 1500]
+    pos, vel = make_systems(sizes, 45)
+    sl = slices(sizes)
+    for k, pattern in enumerate(["equal", "few_heavy", "loguniform", "one_heavy"]):
+        pos[sl[k]] = mass_pattern(pos[sl[k]], pattern, seed=k)
+    tgt = U.generate(U.GEN_TARGETS, sum(sizes), 9, 0.5)
+    say(f"case: batched mass patterns tile={tile}")
+    with U.Batch(sizes, eps2, dt, tile=tile) as b:
+        b.set_state(pos, vel)
+        b.run(steps, mode=0)
+        gp, _ = b.get_state()
+        b.set_state(pos, vel)
+        _, losses, dx, dv, dm = b.run(steps, mode=0, ckpt_every=ck, target3=tgt)
+    for k in range(len(sizes)):
+        p64, _ = O.steps_f64(pos[sl[k]], vel[sl[k]], steps, eps2, dt)
+        assert relnorm(gp[sl[k], :3], p64[:, :3]) <= 1e-9, k
+        with U.System(sizes[k], U.PREC_FP64, eps2, dt) as s:
+            s.set_state(pos[sl[k]], vel[sl[k]])
+            L, gx, gv, gm = s.adjoint(steps, ck, tgt[sl[k]])
+        assert losses[k] == pytest.approx(L, rel=1e-9), k
+        assert relnorm(dx[sl[k]], gx) <= 1e-5, k
+        assert relnorm(dv[sl[k]], gv) <= 1e-5, k
+        assert relnorm(dm[sl[k]], gm) <= 1e-5, k
 
 
 def test_config5_full_batch_graph_vs_per_system():
