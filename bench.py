@@ -139,6 +139,7 @@ def cpu_baseline(c, n, threads, rows):
         "cores": threads,
         "host_cpus": os.cpu_count(),
         "cpu_model": model,
+        "threads_pinned": True,  # worker t on the t-th CPU of the process's affinity set
         "kind": "port",
         "sample": f"one force evaluation of rows 0..{rows - 1} against all {n} sources "
                   f"(FP32, reference order) = {rows * float(n):.3g} pair interactions, "

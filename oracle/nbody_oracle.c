@@ -11,10 +11,12 @@
  *
  * Citations are relative to the reference's utc-0.1/ tree.
  */
+#define _GNU_SOURCE
 #include "nbody_oracle.h"
 
 #include <math.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -91,19 +93,43 @@ void orc_gravitation_f64(int n, const double* pos4, double eps2, double* force3)
 
 /* Threaded rows.  Body partition follows
  * benchapps/nbody/utc-snode-mgpu/mgpu/body_task_mgpu.cu:41-49: with T workers
- * and c rows, the first c%T workers take c/T+1 rows, the rest c/T. */
+ * and c rows, the first c%T workers take c/T+1 rows, the rest c/T.  Worker t
+ * is pinned to the t-th CPU of the process's affinity set (round robin). */
 typedef struct {
     int n, i0, i1;
     const float* pos4;
     float eps2;
     float* force3;
+    int cpu; /* CPU to pin this worker to, -1: leave it to the scheduler */
 } rows_job;
 
 static void* rows_worker(void* p)
 {
     rows_job* jb = (rows_job*)p;
+    if (jb->cpu >= 0) {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        CPU_SET(jb->cpu, &set);
+        (void)pthread_setaffinity_np(pthread_self(), sizeof(set), &set); /* best effort */
+    }
     grav_rows_f32(jb->n, jb->pos4, jb->eps2, jb->i0, jb->i1, jb->force3);
     return NULL;
+}
+
+/* The k-th CPU this process may run on (round robin), or -1. */
+static int allowed_cpu(int k)
+{
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) != 0)
+        return -1;
+    int count = CPU_COUNT(&set);
+    if (count <= 0)
+        return -1;
+    k %= count;
+    for (int c = 0; c < CPU_SETSIZE; ++c)
+        if (CPU_ISSET(c, &set) && k-- == 0)
+            return c;
+    return -1;
 }
 
 void orc_gravitation_rows_f32(int n, const float* pos4, float eps2, int i0, int i1,
@@ -128,6 +154,7 @@ void orc_gravitation_rows_f32(int n, const float* pos4, float eps2, int i0, int 
         jobs[t].pos4 = pos4;
         jobs[t].eps2 = eps2;
         jobs[t].force3 = force3;
+        jobs[t].cpu = allowed_cpu(t);
         pthread_create(&th[t], NULL, rows_worker, &jobs[t]);
     }
     for (int t = 0; t < nthreads; ++t)
