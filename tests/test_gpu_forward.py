@@ -202,7 +202,7 @@ def test_every_tile_field_accuracy(prec, tile):
 
 
 @pytest.mark.parametrize("pattern", ["equal", "one_heavy", "few_heavy", "loguniform"])
-@pytest.mark.parametrize("name", ["sym2r32", "sym4r32", "sym8r32", "sym8r4"])
+@pytest.mark.parametrize("name", ["sym2r32", "sym4r32", "sym8r32", "sym8r64", "sym8r4"])
 def test_symmetric_tiles_mass_patterns(name, pattern):
     n = 6000  # not a multiple of 512: the last wave block holds padding
     c = plummer(n, eps=0.005)

@@ -24,21 +24,22 @@ const Tile kTiles[] = {
 int tile_count() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 const Tile& tile_get(int id) { return kTiles[id]; }
 
-// Default tile, from sweeps of the tiles on MI355X (mixed precision, ms/step,
-// best three; Plummer spheres):
-//   N = 262144: sym8r32 14.24, sym8r16 14.50, sym8r8 15.02   (k2s8, one-pass: 20.7)
-//   N = 131072: sym8r32 3.651, sym8r16 3.773, sym8r8 3.937
-//   N =  65536: sym8r32 0.974, sym8r16 1.020, sym4r64 1.052
-//   N =  32768: sym8r32 0.287, sym4r32 0.302, sym4r16 0.311
-//   N =  16384: sym4r32 0.103, sym4r64 0.109, sym8r32 0.116  (k1s16: 0.116)
-//   N =   8192: sym4r32 0.051, sym2r32 0.052, sym4r64 0.056  (k1s16: 0.061)
-//   N =   4096: sym2r32 0.031, k1s16 0.032
-// (sym8r64, added later, runs within 0.5% of sym8r32 from N = 32768: 14.15 vs
-// 14.21 ms at N = 262144, 0.970 vs 0.974 at 65536 -- inside run-to-run noise,
-// so the default stays sym8r32 with half the target-side workspace.)
+// Default tile, from sweeps of every tile on MI355X (mixed precision, ms/step,
+// best three; Plummer spheres, equal masses; log-uniform masses rank alike):
+//   N = 262144: sym8r64 12.29, sym8r48 12.34, sym8r32 12.34   (k2s8, one-pass: 20.7)
+//   N = 131072: sym8r48 3.139, sym8r64 3.143, sym8r32 3.160
+//   N =  65536: sym8r64 0.830, sym8r48 0.830, sym8r32 0.871
+//   N =  32768: sym8r64 0.235, sym8r48 0.239, sym8r32 0.240
+//   N =  16384: sym4r64 0.0758, sym4r32 0.0775, sym4r48 0.0783
+//   N =   8192: sym4r128 0.0321, sym2r32 0.0324, sym4r64 0.0327
+//   N =   4096: sym2r32 0.0170, sym2r64 0.0170, sym4r64 0.0180  (k1s16: 0.0315)
+//   N =   2048: sym2r32 0.0116, sym2r64 0.0128, sym4r32 0.0157  (k1s16: 0.0173)
+//   N =   1024: k1s16 0.0101, sym2r32 0.0112
+// (sym8r48 and sym8r24, tried for a whole number of wave rounds, were not
+// faster and are not in the table.)
 // The symmetric tiles need enough wave blocks to fill the chip (K = 8 from 128
-// bodies per CU, K = 4 from 32), and their reaction partials must fit the
-// 16 GiB workspace limit; above it the best one-pass tile (k2s8) is used.
+// bodies per CU, K = 4 from 32), and their partial sums must fit the device's
+// budget (sym_partial_budget); above it the best one-pass tile (k2s8) is used.
 size_t sym_partial_budget(size_t device_bytes)
 {
     if (device_bytes == 0)
@@ -52,13 +53,13 @@ int tile_default(int64_t n, int cu_count, size_t partial_budget)
     const int64_t n_pad = pad_j(n);
     const auto fits = [&](int k) {
         return n_pad <= (1 << 30) && n_pad % (128 * k) == 0 &&
-               (sym_jpart_floats((int)n_pad, k) + sym_ipart_floats((int)n_pad, k, 32)) * sizeof(float) <=
+               (sym_jpart_floats((int)n_pad, k) + sym_ipart_floats((int)n_pad, k, 64)) * sizeof(float) <=
                    partial_budget;
     };
     // from a sweep of every tile at N = 1024 ... 262144 (equal and log-uniform
     // masses) on MI355X: each choice within 2% of the fastest tile at its size
     if (n >= 128 * cus && fits(8))
-        return 17;  // sym8r32
+        return 22;  // sym8r64
     if (n >= 64 * cus && fits(4))
         return 18;  // sym4r64
     if (n >= 32 * cus && fits(4))
