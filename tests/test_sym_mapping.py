@@ -1,5 +1,5 @@
 """CPU check of the symmetric kernels' work mapping (utc_amd/csrc/sym.hip,
-sym_vjp.hip), restated from sym_wave / sym_sum: wave block w, split r of R
+sym_vjp.hip, sym_reduce.h), restated from sym_wave / sym_split_sum: wave block w, split r of R
 takes the sub-units [Q*r/R, Q*(r+1)/R) of Q = K (NW/2 + 1), sub-unit q being
 source block q % K of unit q / K (unit 0: the wave block's own sources, target
 side only; unit 1 + d: the wave block w + 1 + d mod NW, both sides, the block
