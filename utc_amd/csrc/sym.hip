@@ -28,6 +28,7 @@
 // partials go to HBM, and a second kernel adds them per body in a fixed order
 // (deterministic) and applies the integrator epilogue.
 #include "force_tmpl.h"
+#include "sym_items.h"
 #include "sym_reduce.h"
 
 namespace utcnb {
@@ -188,23 +189,6 @@ __global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict
     if (gw >= NW * R)
         return;  // whole waves only
     sym_wave<K, NEWTON>(src, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
-}
-
-// The next item of a wave.  Static schedule: the items [start[gw], start[gw+1])
-// in turn.  Dynamic schedule (g.counter set): lane 0 takes the next index of
-// the longest-first list with one atomic add; every wave stops at g.nitems,
-// so all waves end once the list is used up.
-__device__ __forceinline__ int batch_next_item(const BatchSymGeom& g, int gw, int& pos, int lane)
-{
-    if (!g.counter) {
-        const int it = g.start[gw] + pos++;
-        return it < g.start[gw + 1] ? it : -1;
-    }
-    int it = 0;
-    if (lane == 0)
-        it = atomicAdd(g.counter, 1);
-    it = __shfl(it, 0, 64);
-    return it < g.nitems ? it : -1;
 }
 
 // Batched systems: each wave runs items (system, wave block, split), a system

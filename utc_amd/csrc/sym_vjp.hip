@@ -25,6 +25,7 @@
 // Partials go to HBM (float4 per body: x, y, z, m) and a second kernel adds
 // them per body in FP64, in a fixed order, into xbar and mbar.
 #include "internal.h"
+#include "sym_items.h"
 #include "sym_reduce.h"
 #include "vjp_pair.h"
 
@@ -227,21 +228,8 @@ __global__ __launch_bounds__(256) void sym_vjp_kernel(const float4* __restrict__
     sym_vjp_wave<K>(jrec, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
 }
 
-// Batched systems: each wave runs items, in the schedule batch_next_item
-// describes (sym.hip).
-__device__ __forceinline__ int batch_vjp_next_item(const BatchSymGeom& g, int gw, int& pos, int lane)
-{
-    if (!g.counter) {
-        const int it = g.start[gw] + pos++;
-        return it < g.start[gw + 1] ? it : -1;
-    }
-    int it = 0;
-    if (lane == 0)
-        it = atomicAdd(g.counter, 1);
-    it = __shfl(it, 0, 64);
-    return it < g.nitems ? it : -1;
-}
-
+// Batched systems: each wave runs items in the schedule of batch_next_item
+// (sym_items.h).
 template <int K>
 __global__ __launch_bounds__(256) void batch_sym_vjp_kernel(BatchSymGeom g, int nwaves,
                                                             const float4* __restrict__ jrec, float eps2,
@@ -253,7 +241,7 @@ __global__ __launch_bounds__(256) void batch_sym_vjp_kernel(BatchSymGeom g, int 
         return;  // whole waves only
     const int lane = threadIdx.x & 63;
     int pos = 0;
-    for (int it = batch_vjp_next_item(g, gw, pos, lane); it >= 0; it = batch_vjp_next_item(g, gw, pos, lane)) {
+    for (int it = batch_next_item(g, gw, pos, lane); it >= 0; it = batch_next_item(g, gw, pos, lane)) {
         const int4 item = g.items[it];  // (system, wave block, split, splits)
         const int s = item.x;
         sym_vjp_wave<K>(jrec + 2 * (size_t)g.off[s], g.nw[s], item.w, item.y, item.z, eps2,
