@@ -77,7 +77,8 @@ def test_reference_config1_bit_exact(dtype):
 def test_reference_inverse_distance_exhaustive():
     """The reference kernels' inverse distance for eps2 >= 2^-96 (ref_inv.h):
     every float d2 in [2^-96, inf] and NaN against the compiler's correctly
-    rounded (float)(1.0 / (double)sqrtf(d2)) on the device, bit for bit."""
+    rounded (float)(1.0 / (double)sqrtf(d2)) on the device, bit for bit; the
+    FP64 one against 1.0 / sqrt(d2) on 2^32 random d2 in [2^-600, 2^600]."""
     import json
     import subprocess
     exe = Path(U.__file__).resolve().parent / "lib" / "probe_invdist"
@@ -93,6 +94,9 @@ def test_reference_inverse_distance_exhaustive():
     # v_rcp_f64 close enough that one Newton step lands within 2^-48
     assert out["sqrt_f32_max_ulps"] <= 1
     assert out["rcp_f64_max_rel_err"] ** 2 + 2.0 ** -53 < 2.0 ** -48
+    # FP64 (ref_inv_dist64): 2^32 random d2 over [2^-600, 2^600] and both ends
+    assert out["f64_checked"] == 2 ** 32 + 2
+    assert out["f64_mismatches"] == 0
 
 
 @pytest.mark.parametrize("eps2", [1e-30, 1e-4])
@@ -114,6 +118,26 @@ def test_reference_f32_far_and_tiny_softening_bit_exact(eps2):
         a = s.accel()
     f = O.gravitation_f32(pos, e2)
     assert np.array_equal(a.astype(np.float32), f, equal_nan=True)
+
+
+@pytest.mark.parametrize("far", [1e100, 1e160])
+def test_reference_f64_far_pairs_bit_exact(far):
+    """FP64 reference arithmetic with pairs whose d2 leaves [2^-600, 2^600]
+    (1e100 apart: d2 = 1e200; 1e160 apart: d2 overflows to inf, inverse
+    distance 0): those groups of four sources take the generic sqrt and
+    division, the rest the short sequence; the field equals the oracle's."""
+    rng = np.random.default_rng(4)
+    n = 1000
+    pos = np.zeros((n, 4))
+    pos[:, :3] = rng.uniform(-1, 1, (n, 3))
+    pos[:, 3] = rng.uniform(0.5, 2, n)
+    pos[:7, 2] += far
+    say(f"case: REFERENCE f64 field with pairs {far:g} apart")
+    with U.System(n, U.PREC_REFERENCE, 1e-4, 1e-3, 1.0, kick=U.KICK_REFERENCE, state_dtype=U.F64) as s:
+        s.set_state(pos, np.zeros_like(pos))
+        a = s.accel()
+    f = O.gravitation_f64(pos, 1e-4)
+    assert np.array_equal(a, f, equal_nan=True)
 
 
 def test_reference_f32_field_matches_oracle_gravitation():

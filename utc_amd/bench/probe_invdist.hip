@@ -7,6 +7,9 @@
 // the raw v_rcp_f64 and v_sqrt_f32 approximations over the same inputs (the
 // margins the two correction steps rely on).  One thread per bit pattern in a
 // grid-stride loop over a 32-bit range; counters via global atomics.
+// The FP64 variant (ref_inv_dist64) is compared with 1.0 / sqrt(d2) on 2^32
+// pseudo-random d2 spread evenly over the binades of [2^-600, 2^600) with
+// random significands (splitmix64 of the sample index), plus both ends.
 // Build: python -m utc_amd.build; run: utc_amd/lib/probe_invdist.
 #include <hip/hip_runtime.h>
 
@@ -27,6 +30,9 @@
 struct Result {
     unsigned long long bad;
     unsigned long long checked;
+    unsigned long long bad64;
+    unsigned long long checked64;
+    unsigned long long first64;
     unsigned int first[8];
     unsigned int nfirst;
     unsigned int rcp_err_bits;   // max |v_rcp_f64(s) * s - 1| as float bits
@@ -71,6 +77,45 @@ __global__ void check(uint32_t lo, uint32_t hi, Result* r)
     atomicMax(&r->sqrt_err_ulps, sq_ulps);
 }
 
+__device__ __attribute__((noinline)) double reference_inv64(double d2)
+{
+    return 1.0 / sqrt(d2);
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Sample k: binade -600 + k % 1200, significand bits from splitmix64(k).
+__device__ __forceinline__ double sample64(uint64_t k)
+{
+    const int e = -600 + (int)(k % 1200);
+    const uint64_t mant = splitmix64(k) & 0x000FFFFFFFFFFFFFull;
+    const uint64_t bits = ((uint64_t)(e + 1023) << 52) | mant;
+    return __longlong_as_double((long long)bits);
+}
+
+__global__ void check64(uint64_t count, Result* r)
+{
+    unsigned long long bad = 0, n = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count + 2; k += stride) {
+        const double x = k < count ? sample64(k) : (k == count ? utcnb::kRefInv64Min : utcnb::kRefInv64Max);
+        ++n;
+        const double want = reference_inv64(x), got = utcnb::ref_inv_dist64(x);
+        if (__double_as_longlong(want) != __double_as_longlong(got)) {
+            ++bad;
+            atomicMax(&r->first64, (unsigned long long)__double_as_longlong(x));
+        }
+    }
+    atomicAdd(&r->bad64, bad);
+    atomicAdd(&r->checked64, n);
+}
+
 int main()
 {
     Result* d = nullptr;
@@ -82,6 +127,9 @@ int main()
     CHK(hipGetLastError());
     hipLaunchKernelGGL(check, dim3(1), dim3(64), 0, 0, 0x7FC00000u, 0x7FC00001u, d);  // quiet NaN
     CHK(hipGetLastError());
+    const uint64_t count64 = 1ull << 32;
+    hipLaunchKernelGGL(check64, dim3(8192), dim3(256), 0, 0, count64, d);
+    CHK(hipGetLastError());
     Result h{};
     CHK(hipMemcpy(&h, d, sizeof(Result), hipMemcpyDeviceToHost));
     CHK(hipFree(d));
@@ -90,6 +138,11 @@ int main()
     std::printf("{\"checked\": %llu, \"mismatches\": %llu, \"first\": [", h.checked, h.bad);
     for (unsigned int k = 0; k < h.nfirst && k < 8; ++k)
         std::printf("%s\"0x%08x\"", k ? ", " : "", h.first[k]);
-    std::printf("], \"rcp_f64_max_rel_err\": %.3e, \"sqrt_f32_max_ulps\": %u}\n", rcp_err, h.sqrt_err_ulps);
-    return h.bad == 0 && h.checked == (unsigned long long)(hi - lo) + 1 ? 0 : 1;
+    std::printf("], \"rcp_f64_max_rel_err\": %.3e, \"sqrt_f32_max_ulps\": %u, \"f64_checked\": %llu, "
+                "\"f64_mismatches\": %llu, \"f64_largest_mismatch_bits\": \"0x%016llx\"}\n",
+                rcp_err, h.sqrt_err_ulps, h.checked64, h.bad64, h.first64);
+    return h.bad == 0 && h.checked == (unsigned long long)(hi - lo) + 1 && h.bad64 == 0 &&
+                   h.checked64 == count64 + 2
+               ? 0
+               : 1;
 }

@@ -10,7 +10,10 @@
 //
 // Source bodies are uniform across the wave and arrive through scalar loads.
 // For eps2 >= 2^-96 the FP32 kernels take ref_inv_dist (ref_inv.h): the same
-// correctly rounded sqrtf and division, without the generic rescaling.
+// correctly rounded sqrtf and division, without the generic rescaling; for
+// eps2 >= 2^-600 the FP64 kernels take ref_inv_dist64 on every group of four
+// sources whose d2 stay below 2^600 (the same operations as the generic FP64
+// sqrt and division there).
 #include "internal.h"
 #include "ref_inv.h"
 
@@ -40,21 +43,71 @@ __device__ __forceinline__ double inv_dist<double, false>(double d2)
     return 1.0 / sqrt(d2);
 }
 
-// One source j in the reference's order of operations.
-template <typename T, typename T4, bool FAST>
-__device__ __forceinline__ void ref_pair(const T4& pj, const T4& pi, T eps2, T& a0, T& a1, T& a2)
+template <>
+__device__ __forceinline__ double inv_dist<double, true>(double d2)
 {
-    T r0 = pj.x - pi.x;
-    T r1 = pj.y - pi.y;
-    T r2 = pj.z - pi.z;
+    // d2 >= eps2 >= kRefInv64Min; beyond kRefInv64Max (or NaN) the generic path
+    if (d2 <= kRefInv64Max)
+        return ref_inv_dist64(d2);
+    return 1.0 / sqrt(d2);
+}
+
+// One source j in the reference's order of operations: the geometry (r and
+// d2 = ((rx rx + ry ry) + rz rz) + eps2), then the sums with inv = 1/sqrt(d2).
+template <typename T, typename T4>
+__device__ __forceinline__ T ref_geom(const T4& pj, const T4& pi, T eps2, T& r0, T& r1, T& r2)
+{
+    r0 = pj.x - pi.x;
+    r1 = pj.y - pi.y;
+    r2 = pj.z - pi.z;
     T d2 = r0 * r0 + r1 * r1 + r2 * r2;
     d2 += eps2;
-    T inv = inv_dist<T, FAST>(d2);
+    return d2;
+}
+
+template <typename T>
+__device__ __forceinline__ void ref_accum(T mj, T inv, T r0, T r1, T r2, T& a0, T& a1, T& a2)
+{
     T inv3 = inv * inv * inv;
-    T s = pj.w * inv3;
+    T s = mj * inv3;
     a0 += r0 * s;
     a1 += r1 * s;
     a2 += r2 * s;
+}
+
+template <typename T, typename T4, bool FAST>
+__device__ __forceinline__ void ref_pair(const T4& pj, const T4& pi, T eps2, T& a0, T& a1, T& a2)
+{
+    T r0, r1, r2;
+    const T d2 = ref_geom(pj, pi, eps2, r0, r1, r2);
+    ref_accum(pj.w, inv_dist<T, FAST>(d2), r0, r1, r2, a0, a1, a2);
+}
+
+// Four FP64 sources with one wave-uniform choice: ref_inv_dist64 when every
+// lane's four d2 lie within its range (always, short of coordinates 1e90
+// apart), else the generic sqrt and division.  Sums in j order as ref_pair.
+__device__ __forceinline__ void ref_quad64(const double4 (&q)[4], const double4& pi, double eps2,
+                                           double& a0, double& a1, double& a2)
+{
+    double r[4][3], d2[4], inv[4];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        d2[k] = ref_geom(q[k], pi, eps2, r[k][0], r[k][1], r[k][2]);
+        ok = ok && d2[k] <= kRefInv64Max;
+    }
+    if (__ballot(!ok) == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            inv[k] = ref_inv_dist64(d2[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            inv[k] = 1.0 / sqrt(d2[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        ref_accum(q[k].w, inv[k], r[k][0], r[k][1], r[k][2], a0, a1, a2);
 }
 
 // Sources in index order.  Four per trip: their loads and inverse distances
@@ -68,10 +121,15 @@ __device__ __forceinline__ void ref_field(const T4* __restrict__ pos, int n, T e
     int j = 0;
     for (; j + 4 <= n; j += 4) {
         const T4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
-        ref_pair<T, T4, FAST>(q0, pi, eps2, a0, a1, a2);
-        ref_pair<T, T4, FAST>(q1, pi, eps2, a0, a1, a2);
-        ref_pair<T, T4, FAST>(q2, pi, eps2, a0, a1, a2);
-        ref_pair<T, T4, FAST>(q3, pi, eps2, a0, a1, a2);
+        if constexpr (FAST && sizeof(T) == 8) {
+            const T4 q[4] = {q0, q1, q2, q3};
+            ref_quad64(q, pi, eps2, a0, a1, a2);
+        } else {
+            ref_pair<T, T4, FAST>(q0, pi, eps2, a0, a1, a2);
+            ref_pair<T, T4, FAST>(q1, pi, eps2, a0, a1, a2);
+            ref_pair<T, T4, FAST>(q2, pi, eps2, a0, a1, a2);
+            ref_pair<T, T4, FAST>(q3, pi, eps2, a0, a1, a2);
+        }
     }
     for (; j < n; ++j)
         ref_pair<T, T4, FAST>(pos[j], pi, eps2, a0, a1, a2);
@@ -138,8 +196,12 @@ hipError_t launch_ref_step_f64(const double4* pos_old, double4* pos_new, double4
 {
     if (block <= 0)
         block = 256;
-    hipLaunchKernelGGL((ref_step_kernel<double, double4, false>), dim3((n + block - 1) / block),
-                       dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
+    if (eps2 >= kRefInv64Min)
+        hipLaunchKernelGGL((ref_step_kernel<double, double4, true>), dim3((n + block - 1) / block),
+                           dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
+    else
+        hipLaunchKernelGGL((ref_step_kernel<double, double4, false>), dim3((n + block - 1) / block),
+                           dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
     return hipGetLastError();
 }
 
@@ -158,8 +220,12 @@ hipError_t launch_ref_accel_f32(const float4* src, int n, float eps2, double* ac
 hipError_t launch_ref_accel_f64(const double4* src, int n, double eps2, double* acc3,
                                 hipStream_t st)
 {
-    hipLaunchKernelGGL((ref_accel_kernel<double, double4, false>), dim3((n + 255) / 256), dim3(256),
-                       0, st, src, n, eps2, acc3);
+    if (eps2 >= kRefInv64Min)
+        hipLaunchKernelGGL((ref_accel_kernel<double, double4, true>), dim3((n + 255) / 256), dim3(256), 0,
+                           st, src, n, eps2, acc3);
+    else
+        hipLaunchKernelGGL((ref_accel_kernel<double, double4, false>), dim3((n + 255) / 256), dim3(256),
+                           0, st, src, n, eps2, acc3);
     return hipGetLastError();
 }
 

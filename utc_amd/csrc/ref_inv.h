@@ -59,4 +59,42 @@ __device__ __forceinline__ float ref_inv_dist(float d2)
     return ref_recip_rn(ref_sqrt_rn(d2));
 }
 
+// FP64: 1.0 / sqrt(d2) (benchapps/nbody/seq/bodysystem.h:86 for T = double)
+// for d2 in [kRefInv64Min, kRefInv64Max].  The generic correctly rounded
+// sequences of this target are, for such arguments, the operations below plus
+// steps that then change nothing: sqrt = Goldschmidt's iteration from
+// v_rsq_f64 (g = x y, h = y/2, r = 1/2 - h g, g += g r, h += h r) and two
+// Newton corrections g += (x - g^2) h, with a rescaling of x below 2^-767 and
+// a special-value select for 0 and inf; the division 1/g = v_rcp_f64 and two
+// Newton steps z += z (1 - g z), q = z and the residual correction
+// q + (1 - g q) z, with v_div_scale / v_div_fmas / v_div_fixup, which rescale
+// only for extreme exponent gaps and handle special values.  Between 2^-600
+// and 2^600, g and 1/g stay hundreds of binades from every such threshold, so
+// the same operations in the same order give the same bits.
+// utc_amd/bench/probe_invdist.hip compares the two on the device.
+constexpr double kRefInv64Min = 0x1p-600, kRefInv64Max = 0x1p600;
+
+__device__ __forceinline__ double ref_inv_dist64(double x)
+{
+#pragma clang fp contract(off)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    double z = __builtin_amdgcn_rcp(g);
+    double e = __builtin_fma(-g, z, 1.0);
+    z = __builtin_fma(z, e, z);
+    e = __builtin_fma(-g, z, 1.0);
+    z = __builtin_fma(z, e, z);
+    const double q = z;  // 1.0 * z
+    const double res = __builtin_fma(-g, q, 1.0);
+    return __builtin_fma(res, z, q);
+}
+
 }  // namespace utcnb
