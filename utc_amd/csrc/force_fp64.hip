@@ -3,9 +3,11 @@
 //
 // Same mapping as the FP32 kernels (sources uniform per wave, scalar loads of
 // double4 bodies into SGPR pairs, S waves splitting the sources), with a
-// correctly rounded sqrt and division per pair and Kahan-compensated chunk
-// sums, so every field component carries only a few FP64 ulps of error.
+// correctly rounded sqrt and division per pair (ref_inv_dist64 where it gives
+// the same bits) and Kahan-compensated chunk sums, so every field component
+// carries only a few FP64 ulps of error.
 #include "internal.h"
+#include "ref_inv.h"
 
 namespace utcnb {
 
@@ -65,16 +67,37 @@ __global__ __launch_bounds__(64 * W) void force64_kernel(const double4* __restri
     const int j0 = part * len;
     for (int jc = j0; jc < j0 + len; jc += kChunk) {
         double ax = 0, ay = 0, az = 0;
-#pragma unroll 4
-        for (int j = jc; j < jc + kChunk; ++j) {
-            const double4 b = src[j];
-            double rx = b.x - pi.x, ry = b.y - pi.y, rz = b.z - pi.z;
-            double d2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, __builtin_fma(rz, rz, eps2)));
-            double inv = 1.0 / __builtin_sqrt(d2);
-            double s = b.w * (inv * inv * inv);
-            ax = __builtin_fma(rx, s, ax);
-            ay = __builtin_fma(ry, s, ay);
-            az = __builtin_fma(rz, s, az);
+        for (int j = jc; j < jc + kChunk; j += 4) {
+            // four sources: their geometry, one wave-uniform choice of the
+            // inverse-distance sequence (ref_inv.h: the generic correctly
+            // rounded one's operations when every d2 is in [2^-600, 2^600],
+            // so the result bits do not depend on the choice), then the sums
+            double r[4][3], d2[4], inv[4], m[4];
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double4 b = src[j + k];
+                r[k][0] = b.x - pi.x; r[k][1] = b.y - pi.y; r[k][2] = b.z - pi.z;
+                m[k] = b.w;
+                d2[k] = __builtin_fma(r[k][0], r[k][0], __builtin_fma(r[k][1], r[k][1], __builtin_fma(r[k][2], r[k][2], eps2)));
+                ok = ok && d2[k] >= kRefInv64Min && d2[k] <= kRefInv64Max;
+            }
+            if (__ballot(!ok) == 0) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    inv[k] = ref_inv_dist64(d2[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    inv[k] = 1.0 / __builtin_sqrt(d2[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double s = m[k] * (inv[k] * inv[k] * inv[k]);
+                ax = __builtin_fma(r[k][0], s, ax);
+                ay = __builtin_fma(r[k][1], s, ay);
+                az = __builtin_fma(r[k][2], s, az);
+            }
         }
         kahan_add64(hx, cx, ax);
         kahan_add64(hy, cy, ay);
