@@ -16,6 +16,7 @@
 // Per pair: 30 VALU ops + one v_rsq (FP32) or a correctly rounded sqrt and
 // division (FP64), with Kahan-compensated 64-source chunks.
 #include "internal.h"
+#include "ref_inv.h"
 #include "vjp_pair.h"
 
 namespace utcnb {
@@ -41,6 +42,10 @@ __device__ __forceinline__ float inv_sqrt<float>(float d2)
 template <>
 __device__ __forceinline__ double inv_sqrt<double>(double d2)
 {
+    // the correctly rounded 1/sqrt; ref_inv_dist64 gives the same bits with
+    // fewer operations when every lane's d2 is in [2^-600, 2^600] (ref_inv.h)
+    if (__ballot(!(d2 >= kRefInv64Min && d2 <= kRefInv64Max)) == 0)
+        return ref_inv_dist64(d2);
     return 1.0 / __builtin_sqrt(d2);
 }
 
