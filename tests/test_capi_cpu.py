@@ -114,6 +114,7 @@ def test_create_validates_before_touching_a_device():
     h = C.c_void_p()
     bad = [
         U.Config(0, U.PREC_MIXED, 1, 1, 0, 0, 1e-4, 1.0, 1e-3),      # n < 1
+        U.Config((1 << 30) + 1, U.PREC_MIXED, 1, 1, 0, 0, 1e-4, 1.0, 1e-3),  # n above 2^30
         U.Config(10, 7, 1, 1, 0, 0, 1e-4, 1.0, 1e-3),                # precision
         U.Config(10, U.PREC_MIXED, 5, 1, 0, 0, 1e-4, 1.0, 1e-3),     # kick
         U.Config(10, U.PREC_MIXED, 1, 1, 0, 0, 0.0, 1.0, 1e-3),      # eps2 = 0
