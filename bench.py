@@ -83,11 +83,11 @@ def read_traffic(path: Path):
     return float(m.group(1)) if m else None
 
 
-def config2_accuracy(U):
+def plummer65536_accuracy(U):
     """N = 65536 Plummer, eps = 0.005, dt = 5e-4, 20 steps: plain FP32 and mixed
     against the on-device FP64 path (acceleration and final positions)."""
-    from tests.nbcases import config2
-    c = config2()
+    from tests.nbcases import plummer65536
+    c = plummer65536()
     n, eps2, steps = len(c["pos"]), c["eps"] ** 2, c["steps"]
     with U.System(n, U.PREC_FP64, eps2, c["dt"]) as s64:
         s64.set_state(c["pos"], c["vel"])
@@ -109,7 +109,7 @@ def config2_accuracy(U):
             out[label] = dict(tile=s.tile()[1], ms_per_step=wall / steps,
                               interactions_per_s=float(n) * n * steps / (wall * 1e-3),
                               accel_relerr=relnorm(a, a64), final_pos_relerr=relnorm(p[:, :3], p64[:, :3]))
-        log(f"config 2 {label}: {out[label]['ms_per_step']:.3f} ms/step, accel rel err "
+        log(f"N=65536 Plummer {label}: {out[label]['ms_per_step']:.3f} ms/step, accel rel err "
             f"{out[label]['accel_relerr']:.3e}, final-position rel err {out[label]['final_pos_relerr']:.3e}")
     return out
 
@@ -164,12 +164,12 @@ def cpu_baseline(c, n, threads, rows):
     return out
 
 
-def batched_config5(U, nsys=256, steps=20, seed=0):
+def batched_256_systems(U, nsys=256, steps=20, seed=0):
     """256 Plummer systems, n ~ round(exp(U[ln 256, ln 8192])), each with its
     own seed, eps = 0.01, dt = 1e-3, 20 steps; forward and forward+adjoint, one
     captured HIP graph of persistent kernels vs per-system launches."""
-    from tests.nbcases import config5_sizes
-    sizes = config5_sizes(nsys, seed)
+    from tests.nbcases import batch256_sizes
+    sizes = batch256_sizes(nsys, seed)
     pos, vel = [], []
     for k, n in enumerate(sizes):
         p, v = U.generate(U.GEN_PLUMMER, n, 1000 + k)
@@ -227,12 +227,12 @@ def reference_arithmetic(U, c, n, steps=3):
     return out
 
 
-def config4_fwd_adjoint(U):
+def cube_logm16384_fwd_adjoint(U):
     """N = 16384 uniform cube, log-uniform masses in [0.5, 2]/N, eps = 0.01,
     dt = 1e-3, 50 steps, checkpoint every 10: the forward alone and forward +
     adjoint (same loss as fwd_adjoint), mixed precision, default tile."""
-    from tests.nbcases import config4
-    c = config4()
+    from tests.nbcases import cube_logm16384
+    c = cube_logm16384()
     n, steps, ckpt = len(c["pos"]), c["steps"], c["ckpt"]
     pairs = float(n) * n
     with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
@@ -259,7 +259,7 @@ def config4_fwd_adjoint(U):
                fwd_adj_ms=adj, forward_equivalent_interactions_per_s=pairs * steps / (adj * 1e-3),
                pair_evaluations=launches, ratio_vs_forward=fwd / adj,
                total_ratio_vs_forward=fwd / adj * launches / steps)
-    log(f"config 4 [{tile}]: forward {fwd:.2f} ms, fwd+adjoint {adj:.2f} ms "
+    log(f"N=16384 cube, log-uniform masses [{tile}]: forward {fwd:.2f} ms, fwd+adjoint {adj:.2f} ms "
         f"({out['ratio_vs_forward']:.3f}x forward-equivalent)")
     return out
 
@@ -345,7 +345,7 @@ def main():
 
     pairs = float(n) * float(n)
     if args.only == "batched":
-        r = batched_config5(U)
+        r = batched_256_systems(U)
         print(json.dumps({"metric": METRIC, "value": r["graph_fwd_interactions_per_s"],
                           "unit": "interactions/s", "n_gpus": 1, "only": "batched", "batched": r}),
               flush=True)
@@ -434,13 +434,13 @@ def main():
                   momentum_end=[float(x) for x in p1])
     log(f"energy drift {energy['rel_drift']:.3e}, virial 2K/|W| {energy['virial_2K_over_W']:.4f}")
 
-    # ---- config 2 accuracy (N = 65536) ----------------------------------------------
+    # ---- N = 65536 Plummer accuracy ------------------------------------------------
     cfg2 = None
     try:
-        cfg2 = config2_accuracy(U)
+        cfg2 = plummer65536_accuracy(U)
     except Exception as e:  # reported, never hidden
         cfg2 = {"error": str(e)}
-        log(f"config 2 failed: {e}")
+        log(f"N=65536 Plummer accuracy failed: {e}")
 
     # ---- forward + adjoint ----------------------------------------------------------
     fwd_adj = None
@@ -464,16 +464,16 @@ def main():
     cfg4 = None
     if not args.no_adjoint:
         try:
-            cfg4 = config4_fwd_adjoint(U)
+            cfg4 = cube_logm16384_fwd_adjoint(U)
         except Exception as e:  # reported, never hidden
             cfg4 = {"error": str(e)}
-            log(f"config 4 failed: {e}")
+            log(f"N=16384 adjoint leg failed: {e}")
 
-    # ---- batched systems (config 5) ------------------------------------------------
+    # ---- 256 batched systems -------------------------------------------------------
     batch = None
     if not args.no_batch:
         try:
-            batch = batched_config5(U)
+            batch = batched_256_systems(U)
         except Exception as e:  # reported, never hidden
             batch = {"error": str(e)}
             log(f"batched run failed: {e}")
@@ -538,10 +538,10 @@ def main():
         },
         "variants": results,
         "accuracy_vs_fp64": acc,
-        "config2_vs_fp64": cfg2,
+        "plummer65536_vs_fp64": cfg2,
         "energy": energy,
         "fwd_adjoint": fwd_adj,
-        "config4_fwd_adjoint": cfg4,
+        "cube_logm16384_fwd_adjoint": cfg4,
         "reference_arithmetic": ref_arith,
         "batched": batch,
         "fraction_of_fp32_peak": achieved / PEAK_FP32,

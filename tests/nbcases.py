@@ -1,14 +1,13 @@
-"""Shared input builders for the tests (the benchmark configurations).
+"""Shared input builders for the tests and the benchmark (seeded workloads):
 
-Configurations (seeded):
-  config 1  N=4096 uniform cube [-1,1]^3, v ~ N(0, 0.01^2), m = 1/N, eps = 0.01,
-            dt = 1e-3, 10 steps, damping 1, reference kick (as the repo runs it)
-  config 2  N=65536 Plummer, eps = 0.005, dt = 5e-4, 20 steps, physical kick
-  config 3  N=262144 Plummer, eps = 0.002, dt = 2.5e-4, 10 steps, physical kick
-  config 4  N=16384 uniform cube, m ~ exp(U[ln .5, ln 2])/N, eps = 0.01,
-            dt = 1e-3, 50 steps, targets ~ N(0, 0.5^2), checkpoint every 10
-  config 5  256 Plummer systems, n ~ round(exp(U[ln 256, ln 8192])), eps = 0.01,
-            dt = 1e-3, 20 steps
+  cube4096        N=4096 uniform cube [-1,1]^3, v ~ N(0, 0.01^2), m = 1/N, eps = 0.01,
+                  dt = 1e-3, 10 steps, damping 1, reference kick (as the repo runs it)
+  plummer65536    N=65536 Plummer, eps = 0.005, dt = 5e-4, 20 steps, physical kick
+  plummer262144   N=262144 Plummer, eps = 0.002, dt = 2.5e-4, 10 steps, physical kick
+  cube_logm16384  N=16384 uniform cube, m ~ exp(U[ln .5, ln 2])/N, eps = 0.01,
+                  dt = 1e-3, 50 steps, targets ~ N(0, 0.5^2), checkpoint every 10
+  batch256_sizes  256 Plummer systems, n ~ round(exp(U[ln 256, ln 8192])), eps = 0.01,
+                  dt = 1e-3, 20 steps
 """
 from __future__ import annotations
 
@@ -20,7 +19,7 @@ SEED = 0
 TARGET_SEED_XOR = 0x5EED7A56E7
 
 
-def config1(n=4096, seed=SEED):
+def cube4096(n=4096, seed=SEED):
     pos, vel = U.generate(U.GEN_CUBE, n, seed, 0.01)
     return dict(pos=pos, vel=vel, eps=0.01, dt=1e-3, steps=10, damping=1.0,
                 kick=U.KICK_REFERENCE)
@@ -32,22 +31,22 @@ def plummer(n, seed=SEED, eps=0.005, dt=5e-4, steps=20):
                 kick=U.KICK_PHYSICAL)
 
 
-def config2(n=65536, seed=SEED):
+def plummer65536(n=65536, seed=SEED):
     return plummer(n, seed, 0.005, 5e-4, 20)
 
 
-def config3(n=262144, seed=SEED):
+def plummer262144(n=262144, seed=SEED):
     return plummer(n, seed, 0.002, 2.5e-4, 10)
 
 
-def config4(n=16384, seed=SEED, steps=50):
+def cube_logm16384(n=16384, seed=SEED, steps=50):
     pos, vel = U.generate(U.GEN_CUBE_LOGM, n, seed, 0.01, 0.5, 2.0)
     tgt = U.generate(U.GEN_TARGETS, n, seed ^ TARGET_SEED_XOR, 0.5)
     return dict(pos=pos, vel=vel, target=tgt, eps=0.01, dt=1e-3, steps=steps, damping=1.0,
                 kick=U.KICK_PHYSICAL, ckpt=10)
 
 
-def config5_sizes(nsys=256, seed=SEED):
+def batch256_sizes(nsys=256, seed=SEED):
     st = np.zeros(4, np.uint64)
     import ctypes as C
     s = (C.c_uint64 * 4)()

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import utc_amd as U
-from nbcases import config4, mass_pattern, relnorm
+from nbcases import cube_logm16384, mass_pattern, relnorm
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -78,11 +78,11 @@ def central_difference(p_plus, p_minus, tgt, h):
 
 
 @pytest.fixture(scope="module")
-def config4_grads():
-    c = config4()
+def cube_logm_grads():
+    c = cube_logm16384()
     n, steps = len(c["pos"]), c["steps"]
     eps2 = c["eps"] ** 2
-    say(f"case: config4 adjoint n={n} steps={steps}")
+    say(f"case: cube_logm16384 adjoint n={n} steps={steps}")
     res = {}
     for prec in (U.PREC_FP64, U.PREC_MIXED):
         with U.System(n, prec, eps2, c["dt"], kick=c["kick"]) as s:
@@ -91,8 +91,8 @@ def config4_grads():
     return c, res
 
 
-def test_config4_gradients_match_fp64_central_differences(config4_grads):
-    """64 random coordinates (positions, velocities, masses) of config 4; the
+def test_cube_logm16384_gradients_match_fp64_central_differences(cube_logm_grads):
+    """64 random coordinates (positions, velocities, masses) of the N=16384 cube; the
     loss of each perturbed input is an FP64 forward on the GPU (itself checked
     against the CPU FP64 oracle in test_gpu_forward.py).  FP64 rounding leaves
     ~1e-13 of noise in the per-body sum of position differences whatever the
@@ -101,7 +101,7 @@ def test_config4_gradients_match_fp64_central_differences(config4_grads):
     body's mass for a mass; the loss is close to quadratic in v and m over 50
     steps, where central differences are exact.  Tolerance: 1e-5 relative per
     coordinate."""
-    c, res = config4_grads
+    c, res = cube_logm_grads
     n, steps = len(c["pos"]), c["steps"]
     eps2 = c["eps"] ** 2
     rng = np.random.default_rng(11)
@@ -136,8 +136,8 @@ def test_config4_gradients_match_fp64_central_differences(config4_grads):
     assert emx.max() <= 1e-5
 
 
-def test_config4_mixed_adjoint_close_to_fp64(config4_grads):
-    c, res = config4_grads
+def test_cube_logm16384_mixed_adjoint_close_to_fp64(cube_logm_grads):
+    c, res = cube_logm_grads
     L64, dx64, dv64, dm64 = res[U.PREC_FP64]
     L, dx, dv, dm = res[U.PREC_MIXED]
     assert L == pytest.approx(L64, rel=1e-9)

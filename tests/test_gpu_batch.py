@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import utc_amd as U
-from nbcases import config5_sizes, mass_pattern, relnorm
+from nbcases import batch256_sizes, mass_pattern, relnorm
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -152,16 +152,16 @@ def test_batched_symmetric_mass_patterns(tile):
         assert relnorm(dm[sl[k]], gm) <= 1e-5, k
 
 
-def test_config5_full_batch_graph_vs_per_system():
-    """Config 5 at full size: 256 Plummer systems (sizes log-uniform in
+def test_batch256_full_graph_vs_per_system():
+    """256 Plummer systems at full size (sizes log-uniform in
     [256, 8192]), 20 steps forward and forward + adjoint.  The tuned graph
     of batched kernels against per-system launches of the single-system
     kernels: positions within 1e-9 (both FP64 state, FP32 pairs), losses
     within 1e-9 and gradients within 1e-5 relative, per system."""
-    sizes = config5_sizes(256, 0)
+    sizes = batch256_sizes(256, 0)
     pos, vel = make_systems(sizes, 1000)
     tgt = U.generate(U.GEN_TARGETS, sum(sizes), 77, 0.5)
-    say(f"case: config5 full batch, {len(sizes)} systems, {sum(sizes)} bodies")
+    say(f"case: 256-system full batch, {len(sizes)} systems, {sum(sizes)} bodies")
     with U.Batch(sizes, 1e-4, 1e-3, tile=0) as bg, U.Batch(sizes, 1e-4, 1e-3, tile=1) as bp:
         bg.set_state(pos, vel)
         bg.run(20, mode=0)

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import utc_amd as U
-from nbcases import config1, config2, config3, mass_pattern, plummer, relnorm
+from nbcases import cube4096, plummer65536, plummer262144, mass_pattern, plummer, relnorm
 from oracle import oracle as O
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -54,13 +54,13 @@ def test_reference_f32_reproduces_golden_file(tmp_path):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_reference_config1_bit_exact(dtype):
-    c = config1()
+def test_reference_cube4096_bit_exact(dtype):
+    c = cube4096()
     T = dtype
     pos = c["pos"].astype(T)
     vel = c["vel"].astype(T)
     eps2 = T(c["eps"]) * T(c["eps"])
-    say(f"case: REFERENCE {np.dtype(T).name} config1 n={len(pos)}")
+    say(f"case: REFERENCE {np.dtype(T).name} cube4096 n={len(pos)}")
     with U.System(len(pos), U.PREC_REFERENCE, float(eps2), float(T(c["dt"])), 1.0,
                   kick=U.KICK_REFERENCE, state_dtype=U.F32 if T == np.float32 else U.F64) as s:
         s.set_state(pos, vel)
@@ -154,7 +154,7 @@ def test_reference_f32_field_matches_oracle_gravitation():
 def test_kernel_level_entry_point_matches_oracle():
     """utcnb_integrate_f32 on device pointers, like the reference's kernel launch."""
     import torch
-    c = config1(n=2048)
+    c = cube4096(n=2048)
     pos = c["pos"].astype(np.float32)
     vel = c["vel"].astype(np.float32)
     eps2 = np.float32(c["eps"]) * np.float32(c["eps"])
@@ -177,17 +177,17 @@ def test_kernel_level_entry_point_matches_oracle():
         assert np.array_equal(v.cpu().numpy(), ov)
 
 
-def test_fp32_config1_within_tolerance_of_reference():
-    """Plain fast FP32 (rsq, FMA) vs the reference CPU path on config 1:
+def test_fp32_cube4096_within_tolerance_of_reference():
+    """Plain fast FP32 (rsq, FMA) vs the reference CPU path on the N=4096 cube:
     max |dx| <= 1e-6 and relative norm <= 1e-6."""
-    c = config1()
+    c = cube4096()
     pos = c["pos"].astype(np.float32)
     vel = c["vel"].astype(np.float32)
     eps2 = np.float32(c["eps"]) * np.float32(c["eps"])
     dt = np.float32(c["dt"])
     op, _ = O.run_f32(pos, vel, c["steps"], eps2, 1.0, dt, nthreads=8)
     for flags in (0, U.FLAG_NO_NEWTON):
-        say(f"case: FP32 config1 flags={flags}")
+        say(f"case: FP32 cube4096 flags={flags}")
         with U.System(len(pos), U.PREC_FP32, float(eps2), float(dt), 1.0,
                       kick=U.KICK_REFERENCE, flags=flags) as s:
             s.set_state(pos, vel)
@@ -256,7 +256,7 @@ def test_odd_sizes(n):
         assert np.abs(gp - p64).max() <= tol * max(1.0, np.abs(p64).max()), (prec, n)
 
 
-def test_mixed_and_fp64_trajectory_config2_small():
+def test_mixed_and_fp64_trajectory_plummer_small():
     """Config-2 physics at N=4096: mixed path final positions vs the FP64 oracle."""
     c = plummer(4096)
     say("case: MIXED/FP64 trajectory n=4096 steps=20")
@@ -287,14 +287,14 @@ def test_diagnostics_match_oracle():
         assert np.abs(p - P).max() <= 1e-15
 
 
-@pytest.mark.parametrize("which", ["config2", "config3"])
+@pytest.mark.parametrize("which", ["plummer65536", "plummer262144"])
 def test_full_size_accuracy_against_fp64(which):
     """Configs 2 and 3 at full size (N = 65536 and 262144, Plummer): final
     positions of the mixed path (default tile, symmetric) and of plain FP32
     against the on-device FP64 path (itself equal to the CPU FP64 oracle to
     1e-14 at N = 2048, test_fp64_field_matches_cpu_fp64); target <= 1e-6
     relative.  The mixed run's total energy and momentum stay put."""
-    c = config2() if which == "config2" else config3()
+    c = plummer65536() if which == "plummer65536" else plummer262144()
     n, eps2, steps = len(c["pos"]), c["eps"] ** 2, c["steps"]
     say(f"case: {which} full size n={n} steps={steps}")
     with U.System(n, U.PREC_FP64, eps2, c["dt"]) as s64:
