@@ -120,6 +120,30 @@ def test_reference_f32_far_and_tiny_softening_bit_exact(eps2):
     assert np.array_equal(a.astype(np.float32), f, equal_nan=True)
 
 
+@pytest.mark.parametrize("n", [1, 3, 17, 1003])
+def test_reference_ragged_sizes_bit_exact(n):
+    """Sizes that leave remainders after the kernels' 16- and 4-source trips
+    (FP32: 1003 = 62 x 16 + 2 x 4 + 3), both state types, 3 steps, against the
+    oracle's sequential loop over every source."""
+    c = cube4096(n=n)
+    say(f"case: REFERENCE ragged n={n}")
+    for T, sd in ((np.float32, U.F32), (np.float64, U.F64)):
+        pos = c["pos"].astype(T)
+        vel = c["vel"].astype(T)
+        eps2 = T(c["eps"]) * T(c["eps"])
+        with U.System(n, U.PREC_REFERENCE, float(eps2), float(T(c["dt"])), 1.0,
+                      kick=U.KICK_REFERENCE, state_dtype=sd) as s:
+            s.set_state(pos, vel)
+            s.step(3)
+            gp, gv = s.get_state(T)
+        if T == np.float32:
+            op, ov = O.run_f32(pos, vel, 3, eps2, 1.0, T(c["dt"]))
+        else:
+            op, ov = O.run_f64(pos, vel, 3, eps2, 1.0, c["dt"])
+        assert np.array_equal(gp, op), (n, T)
+        assert np.array_equal(gv, ov), (n, T)
+
+
 @pytest.mark.parametrize("far", [1e100, 1e160])
 def test_reference_f64_far_pairs_bit_exact(far):
     """FP64 reference arithmetic with pairs whose d2 leaves [2^-600, 2^600]
