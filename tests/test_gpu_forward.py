@@ -201,6 +201,32 @@ def test_kernel_level_entry_point_matches_oracle():
         assert np.array_equal(v.cpu().numpy(), ov)
 
 
+def test_kernel_level_entry_point_f64_matches_oracle():
+    """utcnb_integrate_f64 (the reference's FTYPE = double build) on device
+    pointers, a size that is not a multiple of the block size."""
+    import torch
+    c = cube4096(n=2050)
+    pos = c["pos"].astype(np.float64)
+    vel = c["vel"].astype(np.float64)
+    eps2 = c["eps"] * c["eps"]
+    say("case: utcnb_integrate_f64 n=2050")
+    dev = torch.device("cuda:0")
+    p_old = torch.from_numpy(pos).to(dev)
+    p_new = torch.empty_like(p_old)
+    v = torch.from_numpy(vel).to(dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    op, ov = O.run_f64(pos, vel, 1, eps2, 1.0, c["dt"])
+    for blocksize in (0, 64, 256):
+        p_new.zero_()
+        v.copy_(torch.from_numpy(vel))
+        rc = U.lib().utcnb_integrate_f64(p_old.data_ptr(), p_new.data_ptr(), v.data_ptr(), len(pos),
+                                         c["dt"], eps2, 1.0, blocksize, stream)
+        U.check(rc, "utcnb_integrate_f64")
+        torch.cuda.synchronize()
+        assert np.array_equal(p_new.cpu().numpy(), op), blocksize
+        assert np.array_equal(v.cpu().numpy(), ov), blocksize
+
+
 def test_fp32_cube4096_within_tolerance_of_reference():
     """Plain fast FP32 (rsq, FMA) vs the reference CPU path on the N=4096 cube:
     max |dx| <= 1e-6 and relative norm <= 1e-6."""
