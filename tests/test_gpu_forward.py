@@ -97,6 +97,9 @@ def test_reference_inverse_distance_exhaustive():
     # FP64 (ref_inv_dist64): 2^32 random d2 over [2^-600, 2^600] and both ends
     assert out["f64_checked"] == 2 ** 32 + 2
     assert out["f64_mismatches"] == 0
+    # and m / sqrt(d2) (the energy kernel) on 2^31 random (m, d2)
+    assert out["f64div_checked"] == 2 ** 31
+    assert out["f64div_mismatches"] == 0
 
 
 @pytest.mark.parametrize("eps2", [1e-30, 1e-4])

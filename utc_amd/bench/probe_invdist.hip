@@ -9,7 +9,9 @@
 // grid-stride loop over a 32-bit range; counters via global atomics.
 // The FP64 variant (ref_inv_dist64) is compared with 1.0 / sqrt(d2) on 2^32
 // pseudo-random d2 spread evenly over the binades of [2^-600, 2^600) with
-// random significands (splitmix64 of the sample index), plus both ends.
+// random significands (splitmix64 of the sample index), plus both ends, and
+// the division num / sqrt(d2) (ref_div64 after ref_sqrt64) on 2^31 random
+// pairs with num over the binades of [2^-400, 2^400).
 // Build: python -m utc_amd.build; run: utc_amd/lib/probe_invdist.
 #include <hip/hip_runtime.h>
 
@@ -33,6 +35,8 @@ struct Result {
     unsigned long long bad64;
     unsigned long long checked64;
     unsigned long long first64;
+    unsigned long long baddiv;
+    unsigned long long checkeddiv;
     unsigned int first[8];
     unsigned int nfirst;
     unsigned int rcp_err_bits;   // max |v_rcp_f64(s) * s - 1| as float bits
@@ -99,6 +103,34 @@ __device__ __forceinline__ double sample64(uint64_t k)
     return __longlong_as_double((long long)bits);
 }
 
+// Division num / sqrt(d2) (the energy kernel's m_j / |r|): num with a random
+// significand over the binades of [2^-400, 2^400), d2 as in check64.
+__device__ __attribute__((noinline)) double reference_div64(double num, double d2)
+{
+    return num / sqrt(d2);
+}
+
+__global__ void check64div(uint64_t count, Result* r)
+{
+    unsigned long long bad = 0, n = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
+        const double x = sample64(k);
+        const uint64_t h = splitmix64(k ^ 0x5DEECE66Dull);
+        const int e = -400 + (int)(h % 800);
+        const double num = __longlong_as_double((long long)(((uint64_t)(e + 1023) << 52) |
+                                                            (splitmix64(h) & 0x000FFFFFFFFFFFFFull)));
+        ++n;
+        const double want = reference_div64(num, x), got = utcnb::ref_div64(num, utcnb::ref_sqrt64(x));
+        if (__double_as_longlong(want) != __double_as_longlong(got)) {
+            ++bad;
+            atomicMax(&r->first64, (unsigned long long)__double_as_longlong(x));
+        }
+    }
+    atomicAdd(&r->baddiv, bad);
+    atomicAdd(&r->checkeddiv, n);
+}
+
 __global__ void check64(uint64_t count, Result* r)
 {
     unsigned long long bad = 0, n = 0;
@@ -130,6 +162,9 @@ int main()
     const uint64_t count64 = 1ull << 32;
     hipLaunchKernelGGL(check64, dim3(8192), dim3(256), 0, 0, count64, d);
     CHK(hipGetLastError());
+    const uint64_t countdiv = 1ull << 31;
+    hipLaunchKernelGGL(check64div, dim3(8192), dim3(256), 0, 0, countdiv, d);
+    CHK(hipGetLastError());
     Result h{};
     CHK(hipMemcpy(&h, d, sizeof(Result), hipMemcpyDeviceToHost));
     CHK(hipFree(d));
@@ -139,10 +174,11 @@ int main()
     for (unsigned int k = 0; k < h.nfirst && k < 8; ++k)
         std::printf("%s\"0x%08x\"", k ? ", " : "", h.first[k]);
     std::printf("], \"rcp_f64_max_rel_err\": %.3e, \"sqrt_f32_max_ulps\": %u, \"f64_checked\": %llu, "
-                "\"f64_mismatches\": %llu, \"f64_largest_mismatch_bits\": \"0x%016llx\"}\n",
-                rcp_err, h.sqrt_err_ulps, h.checked64, h.bad64, h.first64);
+                "\"f64_mismatches\": %llu, \"f64_largest_mismatch_bits\": \"0x%016llx\", "
+                "\"f64div_checked\": %llu, \"f64div_mismatches\": %llu}\n",
+                rcp_err, h.sqrt_err_ulps, h.checked64, h.bad64, h.first64, h.checkeddiv, h.baddiv);
     return h.bad == 0 && h.checked == (unsigned long long)(hi - lo) + 1 && h.bad64 == 0 &&
-                   h.checked64 == count64 + 2
+                   h.checked64 == count64 + 2 && h.baddiv == 0 && h.checkeddiv == countdiv
                ? 0
                : 1;
 }

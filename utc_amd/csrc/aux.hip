@@ -1,6 +1,7 @@
 // aux.hip -- O(N) helper kernels (padding, snapshots, conversions) and the
 // FP64 energy / momentum diagnostics.
 #include "internal.h"
+#include "ref_inv.h"
 
 namespace utcnb {
 
@@ -97,7 +98,36 @@ __global__ __launch_bounds__(256) void energy_pass1(const double4* __restrict__ 
     for (int jc = 0; jc < n; jc += kChunk) {
         const int je = jc + kChunk < n ? jc + kChunk : n;
         double a = 0.0;
-        for (int j = jc; j < je; ++j) {
+        int j = jc;
+        // four sources per trip, one wave-uniform choice between the short
+        // forms of the correctly rounded sqrt and division (ref_inv.h; the
+        // same bits inside their ranges) and the generic ones
+        for (; j + 4 <= je; j += 4) {
+            double d2[4], m[4], t[4];
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double4 q = pos[j + k];
+                double rx = q.x - p.x, ry = q.y - p.y, rz = q.z - p.z;
+                d2[k] = rx * rx + ry * ry + rz * rz + eps2;
+                m[k] = q.w;
+                ok = ok && d2[k] >= kRefInv64Min && d2[k] <= kRefInv64Max && m[k] >= kRefDiv64NumMin &&
+                     m[k] <= kRefDiv64NumMax;
+            }
+            if (__ballot(!ok) == 0) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    t[k] = ref_div64(m[k], ref_sqrt64(d2[k]));
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    t[k] = m[k] / __builtin_sqrt(d2[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                a += (j + k == il) ? 0.0 : t[k];
+        }
+        for (; j < je; ++j) {
             const double4 q = pos[j];
             double rx = q.x - p.x, ry = q.y - p.y, rz = q.z - p.z;
             double d2 = rx * rx + ry * ry + rz * rz + eps2;

@@ -74,7 +74,9 @@ __device__ __forceinline__ float ref_inv_dist(float d2)
 // utc_amd/bench/probe_invdist.hip compares the two on the device.
 constexpr double kRefInv64Min = 0x1p-600, kRefInv64Max = 0x1p600;
 
-__device__ __forceinline__ double ref_inv_dist64(double x)
+// sqrt(x) for x in [kRefInv64Min, kRefInv64Max]: the generic sequence's
+// operations without its rescaling and special-value select.
+__device__ __forceinline__ double ref_sqrt64(double x)
 {
 #pragma clang fp contract(off)
     const double y = __builtin_amdgcn_rsq(x);
@@ -87,14 +89,31 @@ __device__ __forceinline__ double ref_inv_dist64(double x)
     g = __builtin_fma(d, h, g);
     d = __builtin_fma(-g, g, x);
     g = __builtin_fma(d, h, g);
-    double z = __builtin_amdgcn_rcp(g);
-    double e = __builtin_fma(-g, z, 1.0);
+    return g;
+}
+
+// num / den for den = ref_sqrt64(x) of such an x and num in
+// [kRefDiv64NumMin, kRefDiv64NumMax] (exponent gap under 700 binades, quotient
+// normal): the generic division's operations without v_div_scale /
+// v_div_fmas / v_div_fixup, which change nothing there.
+constexpr double kRefDiv64NumMin = 0x1p-400, kRefDiv64NumMax = 0x1p400;
+
+__device__ __forceinline__ double ref_div64(double num, double den)
+{
+#pragma clang fp contract(off)
+    double z = __builtin_amdgcn_rcp(den);
+    double e = __builtin_fma(-den, z, 1.0);
     z = __builtin_fma(z, e, z);
-    e = __builtin_fma(-g, z, 1.0);
+    e = __builtin_fma(-den, z, 1.0);
     z = __builtin_fma(z, e, z);
-    const double q = z;  // 1.0 * z
-    const double res = __builtin_fma(-g, q, 1.0);
+    const double q = num * z;
+    const double res = __builtin_fma(-den, q, num);
     return __builtin_fma(res, z, q);
+}
+
+__device__ __forceinline__ double ref_inv_dist64(double x)
+{
+    return ref_div64(1.0, ref_sqrt64(x));
 }
 
 }  // namespace utcnb
