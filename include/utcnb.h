@@ -85,7 +85,7 @@ int utcnb_integrate_f64(const double* pos_old, double* pos_new, double* vel, int
 typedef struct utcnb_system utcnb_system;
 
 typedef struct {
-    int32_t n;           /* bodies, >= 1 */
+    int32_t n;           /* bodies, 1 .. 2^30 */
     int32_t precision;   /* UTCNB_PREC_* */
     int32_t kick;        /* UTCNB_KICK_* */
     int32_t state_dtype; /* UTCNB_PREC_REFERENCE only: UTCNB_F32 or UTCNB_F64 */
