@@ -34,13 +34,17 @@
 namespace utcnb {
 
 // Lane l receives lane l-1's value (lane 0 lane 63's).  Every lane has a
-// source under wave_ror:1, so the "old" operand (0) is never used.  (Passing
-// v as "old" rotates in place and saves a v_mov per rotation, but ran 0.4%
-// slower on MI355X in a same-device A/B of the N = 262144 step.)
+// source under wave_ror:1, so the "old" operand is never used.  Passing v as
+// "old" lets the compiler rotate in place, without a v_mov that first zeroes
+// a new register; which form is faster depends on K (same-device A/B runs on
+// MI355X, equal masses): in place is 7.7% faster at K = 2 (N = 4096, sym2r32)
+// but 3% slower at K = 8 (N = 262144, sym8r64) and 3.4% slower for the
+// batched K = 4 kernel, and the same within noise for the K = 4 tiles.
+template <int K>
 __device__ __forceinline__ float wave_ror1(float v)
 {
     const int x = __float_as_int(v);
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x13C /* wave_ror:1 */, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(K <= 2 ? x : 0, x, 0x13C /* wave_ror:1 */, 0xF, 0xF, false));
 }
 
 // A block of 64 sources (one per lane, s) against the K targets of each lane,
@@ -76,10 +80,10 @@ __device__ __forceinline__ void sym_block(float4 s, const float (&xi)[K], const 
             gy = __builtin_fmaf(-ry, cj, gy);
             gz = __builtin_fmaf(-rz, cj, gz);
         }
-        sx = wave_ror1(sx); sy = wave_ror1(sy); sz = wave_ror1(sz);
+        sx = wave_ror1<K>(sx); sy = wave_ror1<K>(sy); sz = wave_ror1<K>(sz);
         if constexpr (!UNI)
-            sm = wave_ror1(sm);
-        gx = wave_ror1(gx); gy = wave_ror1(gy); gz = wave_ror1(gz);
+            sm = wave_ror1<K>(sm);
+        gx = wave_ror1<K>(gx); gy = wave_ror1<K>(gy); gz = wave_ror1<K>(gz);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -143,7 +147,7 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
                     py[k] = __builtin_fmaf(ry, si, py[k]);
                     pz[k] = __builtin_fmaf(rz, si, pz[k]);
                 }
-                sx = wave_ror1(sx); sy = wave_ror1(sy); sz = wave_ror1(sz); sm = wave_ror1(sm);
+                sx = wave_ror1<K>(sx); sy = wave_ror1<K>(sy); sz = wave_ror1<K>(sz); sm = wave_ror1<K>(sm);
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
