@@ -14,10 +14,12 @@
 // equal; otherwise they differ by the rounding of one product per partial.
 //
 // Mapping.  Bodies form wave blocks of 64*K (lane l holds K targets).  Wave
-// block w takes (a) its own K blocks of 64 sources, target side only, and (b)
-// the source blocks of wave blocks v = w+1 .. w+NW/2 (mod NW), both sides; for
-// even NW the block half way round is taken by the lower half only, so every
-// unordered pair of wave blocks is visited exactly once.  A block of 64
+// block w takes (a) its own K blocks of 64 sources: block kb both sides
+// against target slots k < kb and target side only against slot kb (K <= 4;
+// at K = 8, target side only against every slot), and (b) the source blocks
+// of wave blocks v = w+1 .. w+NW/2 (mod NW), both sides; for even NW the
+// block half way round is taken by the lower half only, so every unordered
+// pair of wave blocks is visited exactly once.  A block of 64
 // sources sits one per lane and rotates around the wave with DPP wave_ror:1
 // for 64 steps, carrying its reaction accumulator, so after 64 steps every
 // target has met every source and each reaction sum is back in its home lane.
@@ -99,6 +101,91 @@ __device__ __forceinline__ void sym_block(float4 s, const float (&xi)[K], const 
     }
 }
 
+// Own source block KB of the wave's block (K <= 4): target slots k < KB meet
+// it two-sided (the pairs of blocks k and KB, each unordered pair once), slot
+// KB one-sided (the block with itself: each ordered pair once, the self pair
+// adds r = 0); slots k > KB take these sources when their own block is the
+// source.  The reaction sums come home to slot KB after 64 rotations.  UNI as
+// in sym_block.
+template <int K, bool NEWTON, bool UNI, int KB>
+__device__ __forceinline__ void sym_own(float4 s, const float (&xi)[K], const float (&yi)[K],
+                                        const float (&zi)[K], const float (&mi)[K], float m, float eps2,
+                                        float (&ax)[K], float (&ay)[K], float (&az)[K], float (&cx)[K],
+                                        float (&cy)[K], float (&cz)[K])
+{
+    float sx = s.x, sy = s.y, sz = s.z, sm = s.w;
+    float px[KB + 1], py[KB + 1], pz[KB + 1];
+#pragma unroll
+    for (int k = 0; k <= KB; ++k)
+        px[k] = py[k] = pz[k] = 0.f;
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll 4
+    for (int t = 0; t < 64; ++t) {
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+            const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+            const float y3 = pair_scale<NEWTON>(d2, 1.0f);
+            const float si = UNI ? y3 : sm * y3;
+            const float cj = UNI ? y3 : mi[k] * y3;
+            px[k] = __builtin_fmaf(rx, si, px[k]);
+            py[k] = __builtin_fmaf(ry, si, py[k]);
+            pz[k] = __builtin_fmaf(rz, si, pz[k]);
+            gx = __builtin_fmaf(-rx, cj, gx);
+            gy = __builtin_fmaf(-ry, cj, gy);
+            gz = __builtin_fmaf(-rz, cj, gz);
+        }
+        {
+            const float rx = sx - xi[KB], ry = sy - yi[KB], rz = sz - zi[KB];
+            const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+            const float si = pair_scale<NEWTON>(d2, UNI ? 1.0f : sm);
+            px[KB] = __builtin_fmaf(rx, si, px[KB]);
+            py[KB] = __builtin_fmaf(ry, si, py[KB]);
+            pz[KB] = __builtin_fmaf(rz, si, pz[KB]);
+        }
+        sx = wave_ror1<K>(sx); sy = wave_ror1<K>(sy); sz = wave_ror1<K>(sz);
+        if constexpr (!UNI)
+            sm = wave_ror1<K>(sm);
+        if constexpr (KB > 0) {
+            gx = wave_ror1<K>(gx); gy = wave_ror1<K>(gy); gz = wave_ror1<K>(gz);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k <= KB; ++k) {
+        if constexpr (UNI) {
+            px[k] *= m; py[k] *= m; pz[k] *= m;
+        }
+        kahan_add(ax[k], cx[k], px[k]);
+        kahan_add(ay[k], cy[k], py[k]);
+        kahan_add(az[k], cz[k], pz[k]);
+    }
+    if constexpr (KB > 0) {
+        if constexpr (UNI) {
+            gx *= m; gy *= m; gz *= m;
+        }
+        kahan_add(ax[KB], cx[KB], gx);
+        kahan_add(ay[KB], cy[KB], gy);
+        kahan_add(az[KB], cz[KB], gz);
+    }
+}
+
+// sym_own for the wave-uniform block index kb, 0 <= kb < K.
+template <int K, bool NEWTON, int KB = 0>
+__device__ __forceinline__ void sym_own_at(int kb, bool uni, float4 s, const float (&xi)[K], const float (&yi)[K],
+                                           const float (&zi)[K], const float (&mi)[K], float m, float eps2,
+                                           float (&ax)[K], float (&ay)[K], float (&az)[K], float (&cx)[K],
+                                           float (&cy)[K], float (&cz)[K])
+{
+    if (kb == KB) {
+        if (uni)
+            sym_own<K, NEWTON, true, KB>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
+        else
+            sym_own<K, NEWTON, false, KB>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
+    } else if constexpr (KB + 1 < K) {
+        sym_own_at<K, NEWTON, KB + 1>(kb, uni, s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
+    }
+}
+
 // The work of wave r of wave block w of one system: src = the system's
 // padded bodies (NW wave blocks of 64K); ipart: [R][n_pad][3] target-side
 // sums; jpart: [NW][NW/2][64K][3] reactions.
@@ -128,8 +215,19 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
     const int q0 = (int)((long long)Q * r / R), q1 = (int)((long long)Q * (r + 1) / R);
     for (int q = q0; q < q1; ++q) {
         const int u = q / K, kb = q - u * K;
+        if constexpr (K <= 4) {
+            if (u == 0) {
+                // own block kb: two-sided against the lower target slots
+                const float4 s = src[w * 64 * K + kb * 64 + lane];
+                const bool uni = __ballot(tuni && s.w == m0) == ~0ull;
+                sym_own_at<K, NEWTON>(kb, uni, s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz);
+                continue;
+            }
+        }
         if (u == 0) {
-            // own block kb: target side only (each ordered pair once, self pair gives 0)
+            // own block kb (K = 8, where own blocks are a small share and
+            // eight variants of sym_own would crowd the instruction cache):
+            // target side only (each ordered pair once, self pair gives 0)
             const float4 s = src[w * 64 * K + kb * 64 + lane];
             float sx = s.x, sy = s.y, sz = s.z, sm = s.w;
             float px[K], py[K], pz[K];

@@ -19,9 +19,10 @@
 // DPP moves).
 //
 // The mapping is the symmetric force kernel's (sym.hip): wave blocks of 64*K
-// bodies, unit 0 = own blocks (target side only), unit 1 + d = the wave block
-// d + 1 places round (both sides), R waves per wave block split the K (NW/2+1)
-// (unit, source block) sub-units.
+// bodies, unit 0 = own blocks (source block kb two-sided against target slots
+// k < kb, one-sided against slot kb), unit 1 + d = the wave block d + 1 places
+// round (both sides), R waves per wave block split the K (NW/2+1) (unit,
+// source block) sub-units.
 // Partials go to HBM (float4 per body: x, y, z, m) and a second kernel adds
 // them per body in FP64, in a fixed order, into xbar and mbar.
 #include "internal.h"
@@ -124,6 +125,116 @@ __device__ __forceinline__ void sym_vjp_block(float4 sp, float4 sa, const float 
     h = make_float4(hx, hy, hz, hm);
 }
 
+// Own source block KB of the wave's block: target slots k < KB meet it
+// two-sided (as sym_vjp_block; no self pairs), slot KB one-sided (the block
+// with itself, self pairs included, so w is exact: vjp_pair.h); slots k > KB
+// take these sources when their own block is the source.  The reaction sums
+// come home to slot KB after 64 rotations.  UNI applies to the two-sided
+// pairs.
+template <int K, bool UNI, int KB>
+__device__ __forceinline__ void sym_vjp_own(float4 sp, float4 sa, const float (&xi)[K], const float (&yi)[K],
+                                            const float (&zi)[K], const float (&mi)[K], const float (&bx)[K],
+                                            const float (&by)[K], const float (&bz)[K], float m, float eps2,
+                                            float (&g)[K][4], float (&c)[K][4])
+{
+    float sx = sp.x, sy = sp.y, sz = sp.z, sm = sp.w, ax = sa.x, ay = sa.y, az = sa.z;
+    float hx = 0.f, hy = 0.f, hz = 0.f, hm = 0.f;  // reaction on the source body
+    float p[KB + 1][4];
+#pragma unroll
+    for (int k = 0; k <= KB; ++k)
+        p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
+#pragma unroll 2
+    for (int t = 0; t < 64; ++t) {
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+            const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+            const float y = __builtin_amdgcn_rsqf(d2);
+            const float y2 = y * y, y3 = y2 * y;
+            const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
+            const float ari = __builtin_fmaf(bx[k], rx, __builtin_fmaf(by[k], ry, bz[k] * rz));
+            float s3, wx, wy, wz;
+            if constexpr (UNI) {
+                s3 = (-3.f * y2) * (arj - ari);
+                wx = ax - bx[k]; wy = ay - by[k]; wz = az - bz[k];
+            } else {
+                s3 = (-3.f * y2) * __builtin_fmaf(mi[k], arj, -(sm * ari));
+                wx = w_fast(mi[k], ax, sm, bx[k]);
+                wy = w_fast(mi[k], ay, sm, by[k]);
+                wz = w_fast(mi[k], az, sm, bz[k]);
+            }
+            const float tx = __builtin_fmaf(s3, rx, wx);
+            const float ty = __builtin_fmaf(s3, ry, wy);
+            const float tz = __builtin_fmaf(s3, rz, wz);
+            p[k][0] = __builtin_fmaf(y3, tx, p[k][0]);
+            p[k][1] = __builtin_fmaf(y3, ty, p[k][1]);
+            p[k][2] = __builtin_fmaf(y3, tz, p[k][2]);
+            hx = __builtin_fmaf(-y3, tx, hx);
+            hy = __builtin_fmaf(-y3, ty, hy);
+            hz = __builtin_fmaf(-y3, tz, hz);
+            p[k][3] = __builtin_fmaf(-y3, arj, p[k][3]);
+            hm = __builtin_fmaf(y3, ari, hm);
+        }
+        {
+            const float rx = sx - xi[KB], ry = sy - yi[KB], rz = sz - zi[KB];
+            const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+            const float y = __builtin_amdgcn_rsqf(d2);
+            const float y2 = y * y, y3 = y2 * y, y5m3 = (-3.f * y2) * y3;
+            const float wx = w_exact(mi[KB], ax, sm, bx[KB]);
+            const float wy = w_exact(mi[KB], ay, sm, by[KB]);
+            const float wz = w_exact(mi[KB], az, sm, bz[KB]);
+            const float q = y5m3 * __builtin_fmaf(rx, wx, __builtin_fmaf(ry, wy, rz * wz));
+            p[KB][0] = __builtin_fmaf(q, rx, __builtin_fmaf(y3, wx, p[KB][0]));
+            p[KB][1] = __builtin_fmaf(q, ry, __builtin_fmaf(y3, wy, p[KB][1]));
+            p[KB][2] = __builtin_fmaf(q, rz, __builtin_fmaf(y3, wz, p[KB][2]));
+            const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
+            p[KB][3] = __builtin_fmaf(-y3, arj, p[KB][3]);
+        }
+        sx = ror1(sx); sy = ror1(sy); sz = ror1(sz); sm = ror1(sm);
+        ax = ror1(ax); ay = ror1(ay); az = ror1(az);
+        if constexpr (KB > 0) {
+            hx = ror1(hx); hy = ror1(hy); hz = ror1(hz); hm = ror1(hm);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k <= KB; ++k) {
+        if constexpr (UNI) {
+            if (k < KB) {
+                p[k][0] *= m; p[k][1] *= m; p[k][2] *= m;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            kahan1(g[k][q], c[k][q], p[k][q]);
+    }
+    if constexpr (KB > 0) {
+        if constexpr (UNI) {
+            hx *= m; hy *= m; hz *= m;
+        }
+        kahan1(g[KB][0], c[KB][0], hx);
+        kahan1(g[KB][1], c[KB][1], hy);
+        kahan1(g[KB][2], c[KB][2], hz);
+        kahan1(g[KB][3], c[KB][3], hm);
+    }
+}
+
+// sym_vjp_own for the wave-uniform block index kb, 0 <= kb < K.
+template <int K, int KB = 0>
+__device__ __forceinline__ void sym_vjp_own_at(int kb, bool uni, float4 sp, float4 sa, const float (&xi)[K],
+                                               const float (&yi)[K], const float (&zi)[K], const float (&mi)[K],
+                                               const float (&bx)[K], const float (&by)[K], const float (&bz)[K],
+                                               float m, float eps2, float (&g)[K][4], float (&c)[K][4])
+{
+    if (kb == KB) {
+        if (uni)
+            sym_vjp_own<K, true, KB>(sp, sa, xi, yi, zi, mi, bx, by, bz, m, eps2, g, c);
+        else
+            sym_vjp_own<K, false, KB>(sp, sa, xi, yi, zi, mi, bx, by, bz, m, eps2, g, c);
+    } else if constexpr (KB + 1 < K) {
+        sym_vjp_own_at<K, KB + 1>(kb, uni, sp, sa, xi, yi, zi, mi, bx, by, bz, m, eps2, g, c);
+    }
+}
+
 // The work of wave r of wave block w of one system.  jrec: two float4 per
 // body {x,y,z,m} {ax,ay,az,0}, n_pad = NW*64*K bodies; ipart: [R][n_pad]
 // float4; jpart: [NW][NW/2][64K] float4.
@@ -159,40 +270,11 @@ __device__ __forceinline__ void sym_vjp_wave(const float4* __restrict__ jrec, in
     for (int su = q0; su < q1; ++su) {
         const int u = su / K, kb = su - u * K;
         if (u == 0) {
-            // own block kb: target side only; it holds the self pairs, so w is exact (vjp_pair.h)
+            // own block kb: two-sided against the lower target slots
             const int b = w * 64 * K + kb * 64 + lane;
             const float4 sp = jrec[2 * b], sa = jrec[2 * b + 1];
-            float sx = sp.x, sy = sp.y, sz = sp.z, sm = sp.w, ax = sa.x, ay = sa.y, az = sa.z;
-            float p[K][4];
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
-#pragma unroll 2
-            for (int t = 0; t < 64; ++t) {
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
-                    const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-                    const float y = __builtin_amdgcn_rsqf(d2);
-                    const float y2 = y * y, y3 = y2 * y, y5m3 = (-3.f * y2) * y3;
-                    const float wx = w_exact(mi[k], ax, sm, bx[k]);
-                    const float wy = w_exact(mi[k], ay, sm, by[k]);
-                    const float wz = w_exact(mi[k], az, sm, bz[k]);
-                    const float q = y5m3 * __builtin_fmaf(rx, wx, __builtin_fmaf(ry, wy, rz * wz));
-                    p[k][0] = __builtin_fmaf(q, rx, __builtin_fmaf(y3, wx, p[k][0]));
-                    p[k][1] = __builtin_fmaf(q, ry, __builtin_fmaf(y3, wy, p[k][1]));
-                    p[k][2] = __builtin_fmaf(q, rz, __builtin_fmaf(y3, wz, p[k][2]));
-                    const float arj = __builtin_fmaf(ax, rx, __builtin_fmaf(ay, ry, az * rz));
-                    p[k][3] = __builtin_fmaf(-y3, arj, p[k][3]);
-                }
-                sx = ror1(sx); sy = ror1(sy); sz = ror1(sz); sm = ror1(sm);
-                ax = ror1(ax); ay = ror1(ay); az = ror1(az);
-            }
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    kahan1(g[k][q], c[k][q], p[k][q]);
+            const bool uni = __ballot(tuni && sp.w == m0) == ~0ull;
+            sym_vjp_own_at<K>(kb, uni, sp, sa, xi, yi, zi, mi, bx, by, bz, m0, eps2, g, c);
             continue;
         }
         const int d = u - 1;
