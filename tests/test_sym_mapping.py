@@ -1,9 +1,11 @@
 """CPU check of the symmetric kernels' work mapping (utc_amd/csrc/sym.hip,
 sym_vjp.hip, sym_reduce.h), restated from sym_wave / sym_split_sum: wave block w, split r of R
 takes the sub-units [Q*r/R, Q*(r+1)/R) of Q = K (NW/2 + 1), sub-unit q being
-source block q % K of unit q / K (unit 0: the wave block's own sources, target
-side only; unit 1 + d: the wave block w + 1 + d mod NW, both sides, the block
-half way round only from the lower half when NW is even); the reduction of
+source block q % K of unit q / K (unit 0: the wave block's own sources; for
+K <= 4 own block kb meets target blocks k < kb both sides and block kb target
+side only (sym_own), for K = 8 every target block target side only; unit 1 + d:
+the wave block w + 1 + d mod NW, both sides, the block half way round only
+from the lower half when NW is even); the reduction of
 body block v reads the reaction partials (w = v - 1 - d mod NW, d, kb).
 Every ordered pair of 64-body blocks must be evaluated exactly once, every
 reaction slot written once and read once."""
@@ -31,6 +33,12 @@ def cover(NW, R, K):
                         continue
                     v = (w + 1 + d) % NW
                     writes[(w, d, kb)] = writes.get((w, d, kb), 0) + 1
+                if u == 0 and K <= 4:
+                    for k in range(kb):         # own block kb, both sides with k < kb
+                        add((w, k, w, kb))
+                        add((w, kb, w, k))
+                    add((w, kb, w, kb))         # and target side with itself
+                    continue
                 for k in range(K):
                     add((w, k, v, kb))          # target side: w's block k from v's block kb
                     if u > 0:
