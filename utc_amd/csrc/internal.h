@@ -148,6 +148,7 @@ struct BatchSymGeom {
     const int* sys_of_block;  // system of each kBatchSymAlign-body block
     int* counter;             // work counter of the dynamic schedule, or null (static lists)
     int nitems;               // items in the dynamic schedule
+    int body0;                // the reductions cover bodies [body0, body0 + total)
 };
 hipError_t launch_batch_counter_reset(int* counter, hipStream_t st);
 hipError_t launch_batch_sym_step(const BatchSymGeom& g, int nwaves, int total, const float4* snap_old,

@@ -350,15 +350,15 @@ __global__ __launch_bounds__(256) void batch_sym_reduce_kernel(BatchSymGeom g, i
                                                                const float* __restrict__ ipart,
                                                                const float* __restrict__ jpart, EpiMixed epi)
 {
-    int b, t;
-    sym_reduce_lane<T>(b, t);
-    const int bb = b < total ? b : total - 1;
+    int j, t;
+    sym_reduce_lane<T>(j, t);
+    const int b = g.body0 + j, bb = g.body0 + (j < total ? j : total - 1);
     const int s = g.sys_of_block[bb / kBatchSymAlign];
     const int loc = bb - g.off[s];  // padding bodies are summed too (their records exist)
     double sum[3];
     sym_split_sum<T>((const Part3*)(ipart + g.ipo[s]), (const Part3*)(jpart + g.jpo[s]), loc, g.nw[s], g.rs[s],
                      K, t, sum);
-    if (b < total && loc < g.n[s] && t == 0)
+    if (j < total && loc < g.n[s] && t == 0)
         sym_epilogue(epi, b, sum);  // padding bodies keep their far-away zero-mass records
 }
 

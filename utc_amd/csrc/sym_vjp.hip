@@ -360,14 +360,14 @@ __global__ __launch_bounds__(256) void batch_sym_vjp_reduce_kernel(BatchSymGeom 
                                                                    double* __restrict__ xbar3,
                                                                    double* __restrict__ mbar)
 {
-    int b, t;
-    sym_reduce_lane<T>(b, t);
-    const int bb = b < total ? b : total - 1;
+    int j, t;
+    sym_reduce_lane<T>(j, t);
+    const int b = g.body0 + j, bb = g.body0 + (j < total ? j : total - 1);
     const int s = g.sys_of_block[bb / kBatchSymAlign];
     const int loc = bb - g.off[s];
     double sum[4];
     sym_split_sum<T>(ipart + g.vipo[s], jpart + g.vjpo[s], loc, g.nw[s], g.rs[s], K, t, sum);
-    if (b < total && loc < g.n[s] && t == 0) {
+    if (j < total && loc < g.n[s] && t == 0) {
         xbar3[3 * (size_t)b + 0] += sum[0];
         xbar3[3 * (size_t)b + 1] += sum[1];
         xbar3[3 * (size_t)b + 2] += sum[2];
