@@ -447,7 +447,9 @@ def main():
     if not args.no_adjoint:
         try:
             fwd_adj = fwd_adjoint(U, c, n, K, results["mixed"]["interactions_per_s"])
-            # every step stored (no recompute): the cost of the reverse sweep alone
+            # a checkpoint every step: nothing to recompute (at K = 10 with checkpoints
+            # every 10 the only segment is the last, whose positions the forward
+            # sweep keeps, so the two legs do the same work)
             fwd_adj["store_every_step"] = fwd_adjoint(U, c, n, K, results["mixed"]["interactions_per_s"],
                                                       ckpt=1)
         except Exception as e:  # reported, never hidden

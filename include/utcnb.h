@@ -141,7 +141,9 @@ int utcnb_diagnostics(utcnb_system* sys, double* kinetic, double* potential,
  * Forward `nsteps` steps from the current state, then the adjoint for
  * L = (1/n) sum_i |x_i(T) - t_i|^2 back to that state.  FP64 checkpoints of
  * (x, v) are kept every `ckpt_every` steps and the steps in between are
- * recomputed during the reverse sweep.  Precision MIXED (FP32 pair VJP) or
+ * recomputed during the reverse sweep, except those of the last segment,
+ * whose positions the forward sweep leaves in the segment buffer (the same
+ * bits a recompute gives).  Precision MIXED (FP32 pair VJP) or
  * FP64.  target3: host FP64 [3n].  Outputs (host, FP64): *loss, dx0[3n],
  * dv0[3n], dm[n].  The system ends in the final (step nsteps) state.
  */

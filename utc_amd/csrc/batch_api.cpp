@@ -547,12 +547,16 @@ hipError_t enqueue_run_sym(utcnb_batch* b, int nsteps, int ckpt, bool adj, int s
         // every run starts from the FP64 state, so a replayed graph is consistent
         ENQ(launch_snapshot(b->pos + b0, b->snap[0] + b0, (int)Tg, sg));
         int c = 0;
+        const int nck = (nsteps + k - 1) / k, slast = (nck - 1) * k;
         for (int t = 0; t < nsteps; ++t) {
             if (adj && t % k == 0) {
                 double4* ck = b->ckpt + 2 * T * (size_t)(t / k);
                 ENQ(hipMemcpyAsync(ck + b0, b->pos + b0, sb, hipMemcpyDeviceToDevice, sg));
                 ENQ(hipMemcpyAsync(ck + T + b0, b->vel + b0, sb, hipMemcpyDeviceToDevice, sg));
             }
+            if (adj && t >= slast)  // the last segment's positions, kept instead of recomputed
+                ENQ(hipMemcpyAsync(b->segpos + T * (size_t)(t - slast) + b0, b->pos + b0, sb,
+                                   hipMemcpyDeviceToDevice, sg));
             ENQ(sym_step_group(b, sk, g, b->snap[c], b->snap[1 - c], b->pos, b->vel, sg));
             c = 1 - c;
         }
@@ -561,20 +565,21 @@ hipError_t enqueue_run_sym(utcnb_batch* b, int nsteps, int ckpt, bool adj, int s
         const int s0g = b->gsys[g], nsg = b->gsys[g + 1] - s0g;
         const BatchGeom lg{b->d_off + s0g, b->d_n + s0g, b->d_chunks + s0g, nullptr, nullptr};
         ENQ(launch_batch_loss(lg, nsg, b->pos, b->tgt, b->xbar, b->vbar, b->mbar, b->loss + s0g, sg));
-        const int nck = (nsteps + k - 1) / k;
         for (int ci = nck - 1; ci >= 0; --ci) {
             const int s0 = ci * k, s1 = (s0 + k < nsteps) ? s0 + k : nsteps;
             const double4* ck = b->ckpt + 2 * T * (size_t)ci;
-            ENQ(hipMemcpyAsync(b->wpos + b0, ck + b0, sb, hipMemcpyDeviceToDevice, sg));
-            ENQ(hipMemcpyAsync(b->wvel + b0, ck + T + b0, sb, hipMemcpyDeviceToDevice, sg));
-            ENQ(launch_snapshot(b->wpos + b0, b->wsnap[0] + b0, (int)Tg, sg));
-            int wc = 0;
-            for (int t = s0; t < s1; ++t) {
-                ENQ(hipMemcpyAsync(b->segpos + T * (size_t)(t - s0) + b0, b->wpos + b0, sb,
-                                   hipMemcpyDeviceToDevice, sg));
-                if (t + 1 < s1) {
-                    ENQ(sym_step_group(b, sk, g, b->wsnap[wc], b->wsnap[1 - wc], b->wpos, b->wvel, sg));
-                    wc = 1 - wc;
+            if (ci != nck - 1) {  // recompute the segment's positions from its checkpoint
+                ENQ(hipMemcpyAsync(b->wpos + b0, ck + b0, sb, hipMemcpyDeviceToDevice, sg));
+                ENQ(hipMemcpyAsync(b->wvel + b0, ck + T + b0, sb, hipMemcpyDeviceToDevice, sg));
+                ENQ(launch_snapshot(b->wpos + b0, b->wsnap[0] + b0, (int)Tg, sg));
+                int wc = 0;
+                for (int t = s0; t < s1; ++t) {
+                    ENQ(hipMemcpyAsync(b->segpos + T * (size_t)(t - s0) + b0, b->wpos + b0, sb,
+                                       hipMemcpyDeviceToDevice, sg));
+                    if (t + 1 < s1) {
+                        ENQ(sym_step_group(b, sk, g, b->wsnap[wc], b->wsnap[1 - wc], b->wpos, b->wvel, sg));
+                        wc = 1 - wc;
+                    }
                 }
             }
             for (int t = s1 - 1; t >= s0; --t) {
@@ -644,32 +649,36 @@ hipError_t enqueue_run(utcnb_batch* b, int nsteps, int ckpt, bool adj, int mode,
     ENQ(launch_snapshot(b->pos, b->snap[0], (int)T, st));
     int c = 0;
     const int k = ckpt < nsteps ? ckpt : nsteps;
+    const int nck = (nsteps + k - 1) / k, slast = (nck - 1) * k;
     for (int t = 0; t < nsteps; ++t) {
         if (adj && t % k == 0) {
             double4* ck = b->ckpt + 2 * T * (size_t)(t / k);
             ENQ(hipMemcpyAsync(ck, b->pos, sb, hipMemcpyDeviceToDevice, st));
             ENQ(hipMemcpyAsync(ck + T, b->vel, sb, hipMemcpyDeviceToDevice, st));
         }
+        if (adj && t >= slast)  // the last segment's positions, kept instead of recomputed
+            ENQ(hipMemcpyAsync(b->segpos + T * (size_t)(t - slast), b->pos, sb, hipMemcpyDeviceToDevice, st));
         ENQ(step(b->snap[c], b->snap[1 - c], b->pos, b->vel));
         c = 1 - c;
     }
     if (!adj)
         return hipSuccess;
     ENQ(launch_batch_loss(b->geom(0), b->nsys, b->pos, b->tgt, b->xbar, b->vbar, b->mbar, b->loss, st));
-    const int nck = (nsteps + k - 1) / k;
     for (int ci = nck - 1; ci >= 0; --ci) {
         const int s0 = ci * k, s1 = (s0 + k < nsteps) ? s0 + k : nsteps;
         const double4* ck = b->ckpt + 2 * T * (size_t)ci;
-        ENQ(hipMemcpyAsync(b->wpos, ck, sb, hipMemcpyDeviceToDevice, st));
-        ENQ(hipMemcpyAsync(b->wvel, ck + T, sb, hipMemcpyDeviceToDevice, st));
-        ENQ(launch_snapshot(b->wpos, b->wsnap[0], (int)T, st));
-        int wc = 0;
-        for (int t = s0; t < s1; ++t) {
-            ENQ(hipMemcpyAsync(b->segpos + T * (size_t)(t - s0), b->wpos, sb,
-                               hipMemcpyDeviceToDevice, st));
-            if (t + 1 < s1) {
-                ENQ(step(b->wsnap[wc], b->wsnap[1 - wc], b->wpos, b->wvel));
-                wc = 1 - wc;
+        if (ci != nck - 1) {  // recompute the segment's positions from its checkpoint
+            ENQ(hipMemcpyAsync(b->wpos, ck, sb, hipMemcpyDeviceToDevice, st));
+            ENQ(hipMemcpyAsync(b->wvel, ck + T, sb, hipMemcpyDeviceToDevice, st));
+            ENQ(launch_snapshot(b->wpos, b->wsnap[0], (int)T, st));
+            int wc = 0;
+            for (int t = s0; t < s1; ++t) {
+                ENQ(hipMemcpyAsync(b->segpos + T * (size_t)(t - s0), b->wpos, sb,
+                                   hipMemcpyDeviceToDevice, st));
+                if (t + 1 < s1) {
+                    ENQ(step(b->wsnap[wc], b->wsnap[1 - wc], b->wpos, b->wvel));
+                    wc = 1 - wc;
+                }
             }
         }
         for (int t = s1 - 1; t >= s0; --t) {

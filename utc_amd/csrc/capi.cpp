@@ -723,13 +723,21 @@ int utcnb_adjoint(utcnb_system* s, int nsteps, int ckpt_every, const double* tar
 
     ADJ_CHECK(hipMemcpyAsync(a.tgt, target3, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
     ADJ_CHECK(hipEventRecord(s->ev0, st));
-    // forward sweep with checkpoints of (x, v) at steps 0, k, 2k, ...
+    // forward sweep with checkpoints of (x, v) at steps 0, k, 2k, ...; the
+    // positions of the last segment go straight to the segment buffer, which
+    // the reverse sweep would otherwise fill by recomputing that segment (the
+    // recompute runs the same kernels on the same state, so the positions are
+    // the same bits)
+    const int slast = (nck - 1) * k;
     for (int t = 0; t < nsteps; ++t) {
         if (t % k == 0) {
             double4* ck = a.ckpt + 2 * n * (size_t)(t / k);
             ADJ_CHECK(hipMemcpyAsync(ck, cur_p64(s), sb, hipMemcpyDeviceToDevice, st));
             ADJ_CHECK(hipMemcpyAsync(ck + n, s->v64, sb, hipMemcpyDeviceToDevice, st));
         }
+        if (t >= slast)
+            ADJ_CHECK(hipMemcpyAsync(a.segpos + n * (size_t)(t - slast), cur_p64(s), sb,
+                                     hipMemcpyDeviceToDevice, st));
         ADJ_CHECK(step_once(s));
         ++launches;
     }
@@ -742,17 +750,19 @@ int utcnb_adjoint(utcnb_system* s, int nsteps, int ckpt_every, const double* tar
         const int s1 = (s0 + k < nsteps) ? s0 + k : nsteps;
         const double4* ck = a.ckpt + 2 * n * (size_t)c;
         int wc = 0;
-        ADJ_CHECK(hipMemcpyAsync(a.wpos[0], ck, sb, hipMemcpyDeviceToDevice, st));
-        ADJ_CHECK(hipMemcpyAsync(a.wvel, ck + n, sb, hipMemcpyDeviceToDevice, st));
-        if (mixed)
-            ADJ_CHECK(launch_snapshot(a.wpos[0], a.wsnap[0], s->n, st));
-        for (int t = s0; t < s1; ++t) {
-            const double4* cur = mixed ? a.wpos[0] : a.wpos[wc];
-            ADJ_CHECK(hipMemcpyAsync(a.segpos + n * (size_t)(t - s0), cur, sb,
-                                     hipMemcpyDeviceToDevice, st));
-            if (t + 1 < s1) {
-                ADJ_CHECK(work_step(s, wc));
-                ++launches;
+        if (c != nck - 1) {  // recompute the segment's positions from its checkpoint
+            ADJ_CHECK(hipMemcpyAsync(a.wpos[0], ck, sb, hipMemcpyDeviceToDevice, st));
+            ADJ_CHECK(hipMemcpyAsync(a.wvel, ck + n, sb, hipMemcpyDeviceToDevice, st));
+            if (mixed)
+                ADJ_CHECK(launch_snapshot(a.wpos[0], a.wsnap[0], s->n, st));
+            for (int t = s0; t < s1; ++t) {
+                const double4* cur = mixed ? a.wpos[0] : a.wpos[wc];
+                ADJ_CHECK(hipMemcpyAsync(a.segpos + n * (size_t)(t - s0), cur, sb,
+                                         hipMemcpyDeviceToDevice, st));
+                if (t + 1 < s1) {
+                    ADJ_CHECK(work_step(s, wc));
+                    ++launches;
+                }
             }
         }
         for (int t = s1 - 1; t >= s0; --t) {
