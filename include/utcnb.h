@@ -143,7 +143,10 @@ int utcnb_diagnostics(utcnb_system* sys, double* kinetic, double* potential,
  * (x, v) are kept every `ckpt_every` steps and the steps in between are
  * recomputed during the reverse sweep, except those of the last segment,
  * whose positions the forward sweep leaves in the segment buffer (the same
- * bits a recompute gives).  Precision MIXED (FP32 pair VJP) or
+ * bits a recompute gives).  With more than one segment the recomputes run on
+ * an internal stream, one segment ahead of the VJP steps on the handle's
+ * stream (the reference kick keeps them on the handle's stream); the call
+ * returns after both.  Precision MIXED (FP32 pair VJP) or
  * FP64.  target3: host FP64 [3n].  Outputs (host, FP64): *loss, dx0[3n],
  * dv0[3n], dm[n].  The system ends in the final (step nsteps) state.
  */

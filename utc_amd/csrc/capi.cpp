@@ -60,6 +60,9 @@ struct utcnb_system {
         int nck = 0, seg = 0;
         double4* ckpt = nullptr;     // nck * 2n (pos, vel)
         double4* segpos = nullptr;   // seg * n
+        double4* segpos2 = nullptr;  // seg * n, second segment buffer (nck > 1)
+        hipStream_t rstream = nullptr;  // segment recomputes beside the VJP steps
+        hipEvent_t evfwd = nullptr, evrec[2] = {nullptr, nullptr}, evvjp = nullptr;
         double4* wpos[2] = {nullptr, nullptr};  // n_pad (FP64 work positions)
         double4* wvel = nullptr;     // n
         float4* wsnap[2] = {nullptr, nullptr};  // n_pad (MIXED work snapshots)
@@ -127,6 +130,15 @@ void release_adjoint(utcnb_system* s)
     auto& a = s->adj;
     dfree(a.ckpt);
     dfree(a.segpos);
+    dfree(a.segpos2);
+    for (hipEvent_t* ev : {&a.evfwd, &a.evrec[0], &a.evrec[1], &a.evvjp}) {
+        if (*ev)
+            (void)hipEventDestroy(*ev);
+        *ev = nullptr;
+    }
+    if (a.rstream)
+        (void)hipStreamDestroy(a.rstream);
+    a.rstream = nullptr;
     dfree(a.wpos[0]);
     dfree(a.wpos[1]);
     dfree(a.wvel);
@@ -632,6 +644,7 @@ static int adjoint_workspace(utcnb_system* s, int nck, int seg)
         return UTCNB_OK;
     release_adjoint(s);
     int rc = dalloc(&a.ckpt, 2 * n * (size_t)nck) || dalloc(&a.segpos, n * (size_t)seg) ||
+             (nck > 1 && dalloc(&a.segpos2, n * (size_t)seg)) ||
              dalloc(&a.wpos[0], np) || dalloc(&a.wpos[1], np) || dalloc(&a.wvel, n) ||
              dalloc(&a.xbar, 3 * n) || dalloc(&a.vbar, 3 * n) || dalloc(&a.mbar, n) ||
              dalloc(&a.tgt, 3 * n) || dalloc(&a.part, (n + 255) / 256 + 1);
@@ -651,6 +664,13 @@ static int adjoint_workspace(utcnb_system* s, int nck, int seg)
         const size_t rb = f32 ? sizeof(float4) : sizeof(double4);
         if (hipMalloc(&a.jrec, 2 * np * rb) != hipSuccess || hipMalloc(&a.fsrc, np * rb) != hipSuccess)
             rc = 1;
+    }
+    if (!rc && nck > 1) {
+        if (hipStreamCreateWithFlags(&a.rstream, hipStreamNonBlocking) != hipSuccess)
+            rc = 1;
+        for (hipEvent_t* ev : {&a.evfwd, &a.evrec[0], &a.evrec[1], &a.evvjp})
+            if (!rc && hipEventCreateWithFlags(ev, hipEventDisableTiming) != hipSuccess)
+                rc = 1;
     }
     if (rc) {
         release_adjoint(s);
@@ -677,14 +697,14 @@ static int adjoint_workspace(utcnb_system* s, int nck, int seg)
 }
 
 // One forward step of the adjoint's work state (wpos/wvel, with wsnap for MIXED).
-static hipError_t work_step(utcnb_system* s, int& wc)
+static hipError_t work_step(utcnb_system* s, int& wc, hipStream_t st)
 {
     auto& a = s->adj;
     hipError_t e;
     if (s->cfg.precision == UTCNB_PREC_MIXED) {
-        e = launch_mixed_step(s->args, a.wsnap[wc], a.wsnap[1 - wc], a.wpos[0], a.wvel, s->stream);
+        e = launch_mixed_step(s->args, a.wsnap[wc], a.wsnap[1 - wc], a.wpos[0], a.wvel, st);
     } else {
-        e = launch_fp64_step(s->args, a.wpos[wc], a.wpos[1 - wc], a.wvel, s->stream);
+        e = launch_fp64_step(s->args, a.wpos[wc], a.wpos[1 - wc], a.wvel, st);
     }
     wc = 1 - wc;
     return e;
@@ -729,6 +749,9 @@ int utcnb_adjoint(utcnb_system* s, int nsteps, int ckpt_every, const double* tar
     // recompute runs the same kernels on the same state, so the positions are
     // the same bits)
     const int slast = (nck - 1) * k;
+    // segment c's positions live in segbuf(c): two buffers, so that the
+    // recompute of segment c - 1 can run beside the VJP steps of segment c
+    auto segbuf = [&](int c) { return (c % 2 == 0) ? a.segpos : a.segpos2; };
     for (int t = 0; t < nsteps; ++t) {
         if (t % k == 0) {
             double4* ck = a.ckpt + 2 * n * (size_t)(t / k);
@@ -736,7 +759,7 @@ int utcnb_adjoint(utcnb_system* s, int nsteps, int ckpt_every, const double* tar
             ADJ_CHECK(hipMemcpyAsync(ck + n, s->v64, sb, hipMemcpyDeviceToDevice, st));
         }
         if (t >= slast)
-            ADJ_CHECK(hipMemcpyAsync(a.segpos + n * (size_t)(t - slast), cur_p64(s), sb,
+            ADJ_CHECK(hipMemcpyAsync(segbuf(nck - 1) + n * (size_t)(t - slast), cur_p64(s), sb,
                                      hipMemcpyDeviceToDevice, st));
         ADJ_CHECK(step_once(s));
         ++launches;
@@ -744,29 +767,50 @@ int utcnb_adjoint(utcnb_system* s, int nsteps, int ckpt_every, const double* tar
     // loss and its seed at the final state
     double* dloss = a.part + (n + 255) / 256;
     ADJ_CHECK(launch_loss_seed(cur_p64(s), a.tgt, s->n, a.xbar, a.vbar, a.mbar, a.part, dloss, st));
-    // reverse sweep, one checkpoint segment at a time
+    // Reverse sweep, one checkpoint segment at a time.  Every segment but the
+    // last is recomputed from its checkpoint on a stream of its own (rs), one
+    // segment ahead of the VJP steps on st: the VJP of segment c waits for
+    // its recompute (evrec), and the recompute of segment c - 2, which reuses
+    // segment c's buffer, waits for that VJP (evvjp).  The reference kick's
+    // field evaluation in the VJP loop shares the force kernels' partial sums
+    // with the recompute, so that kick runs both on st, in the same order.
+    const bool overlap = nck > 1 && !refkick;
+    hipStream_t rs = overlap ? a.rstream : st;
+    auto recompute = [&](int c) -> hipError_t {
+        const int s0 = c * k, s1 = (s0 + k < nsteps) ? s0 + k : nsteps;
+        const double4* ck = a.ckpt + 2 * n * (size_t)c;
+        double4* buf = segbuf(c);
+        hipError_t e = hipMemcpyAsync(a.wpos[0], ck, sb, hipMemcpyDeviceToDevice, rs);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(a.wvel, ck + n, sb, hipMemcpyDeviceToDevice, rs);
+        if (e == hipSuccess && mixed)
+            e = launch_snapshot(a.wpos[0], a.wsnap[0], s->n, rs);
+        int wc = 0;
+        for (int t = s0; t < s1 && e == hipSuccess; ++t) {
+            const double4* cur = mixed ? a.wpos[0] : a.wpos[wc];
+            e = hipMemcpyAsync(buf + n * (size_t)(t - s0), cur, sb, hipMemcpyDeviceToDevice, rs);
+            if (e == hipSuccess && t + 1 < s1) {
+                e = work_step(s, wc, rs);
+                ++launches;
+            }
+        }
+        if (e == hipSuccess && overlap)
+            e = hipEventRecord(a.evrec[c % 2], rs);
+        return e;
+    };
+    if (overlap) {
+        ADJ_CHECK(hipEventRecord(a.evfwd, st));  // checkpoints and force partials free
+        ADJ_CHECK(hipStreamWaitEvent(rs, a.evfwd, 0));
+    }
+    if (nck > 1)
+        ADJ_CHECK(recompute(nck - 2));
     for (int c = nck - 1; c >= 0; --c) {
         const int s0 = c * k;
         const int s1 = (s0 + k < nsteps) ? s0 + k : nsteps;
-        const double4* ck = a.ckpt + 2 * n * (size_t)c;
-        int wc = 0;
-        if (c != nck - 1) {  // recompute the segment's positions from its checkpoint
-            ADJ_CHECK(hipMemcpyAsync(a.wpos[0], ck, sb, hipMemcpyDeviceToDevice, st));
-            ADJ_CHECK(hipMemcpyAsync(a.wvel, ck + n, sb, hipMemcpyDeviceToDevice, st));
-            if (mixed)
-                ADJ_CHECK(launch_snapshot(a.wpos[0], a.wsnap[0], s->n, st));
-            for (int t = s0; t < s1; ++t) {
-                const double4* cur = mixed ? a.wpos[0] : a.wpos[wc];
-                ADJ_CHECK(hipMemcpyAsync(a.segpos + n * (size_t)(t - s0), cur, sb,
-                                         hipMemcpyDeviceToDevice, st));
-                if (t + 1 < s1) {
-                    ADJ_CHECK(work_step(s, wc));
-                    ++launches;
-                }
-            }
-        }
+        if (overlap && c < nck - 1)
+            ADJ_CHECK(hipStreamWaitEvent(st, a.evrec[c % 2], 0));
         for (int t = s1 - 1; t >= s0; --t) {
-            const double4* xs = a.segpos + n * (size_t)(t - s0);
+            const double4* xs = segbuf(c) + n * (size_t)(t - s0);
             const double* acc = nullptr;
             if (refkick) {
                 if (mixed) {
@@ -791,6 +835,13 @@ int utcnb_adjoint(utcnb_system* s, int nsteps, int ckpt_every, const double* tar
                 ADJ_CHECK(launch_vjp_f64((const double4*)a.jrec, s->n, s->n_pad, s->cfg.eps2,
                                          a.xbar, a.mbar, s->cu_count, st));
             ++launches;
+        }
+        if (c >= 2) {
+            if (overlap) {
+                ADJ_CHECK(hipEventRecord(a.evvjp, st));  // segment c's buffer is free again
+                ADJ_CHECK(hipStreamWaitEvent(rs, a.evvjp, 0));
+            }
+            ADJ_CHECK(recompute(c - 2));
         }
     }
     ADJ_CHECK(hipEventRecord(s->ev1, st));
