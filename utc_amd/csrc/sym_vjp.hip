@@ -379,18 +379,19 @@ constexpr int kSymVjpK = 4;
 
 int sym_vjp_splits(int n_pad, int cu_count)
 {
-    // ~16 waves per SIMD, at least one source block per wave, at most 48
-    // splits (a sweep of R = 8..128 on MI355X: 48 is 3.6% faster than 64 for
-    // the adjoint at N = 16384 and the same within noise at 65536; N = 262144
-    // takes R = 16 either way)
+    // ~16 waves per SIMD, at least one source block per wave, at most 32
+    // splits (sweeps of R on MI355X: for the adjoint at N = 16384, 50 steps,
+    // checkpoint every 10, with the segment recomputes beside the VJP steps,
+    // 32 is 2.6% faster than 48 and 0.6% faster than 64; at 65536 16..64 are
+    // the same within noise; N = 262144 takes R = 16 either way)
     const int NW = n_pad / (64 * kSymVjpK);
     const int want = (cu_count > 0 ? cu_count : 256) * 4 * 16;
     int R = (want + NW - 1) / NW;
     const int sub = (NW / 2 + 1) * kSymVjpK;
     if (R > sub)
         R = sub;
-    if (R > 48)
-        R = 48;
+    if (R > 32)
+        R = 32;
     return R < 1 ? 1 : R;
 }
 
