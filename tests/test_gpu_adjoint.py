@@ -52,7 +52,7 @@ def test_checkpoint_interval_does_not_change_gradients(prec):
     say(f"case: checkpoint invariance prec={prec}")
     outs = []
     with U.System(n, prec, eps2, dt) as s:
-        for k in (1, 7, 20):
+        for k in (1, 7, 10, 20):  # 20, 3, 2 and 1 segments
             s.set_state(pos, vel)
             outs.append(s.adjoint(steps, k, tgt))
     for L, dx, dv, dm in outs[1:]:
