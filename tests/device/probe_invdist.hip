@@ -12,7 +12,8 @@
 // random significands (splitmix64 of the sample index), plus both ends, and
 // the division num / sqrt(d2) (ref_div64 after ref_sqrt64) on 2^31 random
 // pairs with num over the binades of [2^-400, 2^400).
-// Build: python -m utc_amd.build; run: utc_amd/lib/probe_invdist.
+// Test infrastructure (run by tests/test_gpu_forward.py).  Build: python -m
+// utc_amd.build; run: tests/device/_bin/probe_invdist.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

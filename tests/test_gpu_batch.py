@@ -194,3 +194,35 @@ def test_batch_rejects_bad_arguments():
         b.set_state(p, v)
         with pytest.raises(U.UtcnbError):
             b.run(2, mode=0, ckpt_every=1, target3=np.zeros((100, 3)))
+
+
+@pytest.mark.parametrize("tile", [2, 4])
+def test_graph_recaptured_after_adjoint_workspace_grows(tile):
+    """A mode 0 adjoint run captures a graph over the adjoint workspace; a
+    later run that needs more checkpoints or a longer segment reallocates that
+    workspace.  The next mode 0 run with the first run's arguments must
+    capture again (not replay launches on the freed buffers) and give the
+    same bits as a fresh batch."""
+    eps2, dt = 1e-4, 1e-3
+    sizes = [256, 700, 1500, 64]
+    pos, vel = make_systems(sizes, 61)
+    tgt = U.generate(U.GEN_TARGETS, sum(sizes), 13, 0.5)
+    say(f"case: graph recapture after workspace growth tile={tile}")
+    with U.Batch(sizes, eps2, dt, tile=tile) as fresh:
+        fresh.set_state(pos, vel)
+        ref = fresh.run(20, mode=0, ckpt_every=10, target3=tgt)[1:]
+        ref_state = fresh.get_state()
+    with U.Batch(sizes, eps2, dt, tile=tile) as b:
+        b.set_state(pos, vel)
+        b.run(20, mode=0, ckpt_every=10, target3=tgt)       # 2 checkpoints, segment 10
+        b.set_state(pos, vel)
+        b.run(30, mode=2, ckpt_every=15, target3=tgt)       # segment 15: workspace grows
+        b.set_state(pos, vel)
+        b.run(45, mode=1, ckpt_every=15, target3=tgt)       # 3 checkpoints: grows again
+        b.set_state(pos, vel)
+        got = b.run(20, mode=0, ckpt_every=10, target3=tgt)[1:]
+        got_state = b.get_state()
+    for r, g in zip(ref, got):
+        assert np.array_equal(r, g)
+    assert np.array_equal(ref_state[0], got_state[0])
+    assert np.array_equal(ref_state[1], got_state[1])

@@ -81,7 +81,7 @@ def test_reference_inverse_distance_exhaustive():
     FP64 one against 1.0 / sqrt(d2) on 2^32 random d2 in [2^-600, 2^600]."""
     import json
     import subprocess
-    exe = Path(U.__file__).resolve().parent / "lib" / "probe_invdist"
+    exe = Path(__file__).resolve().parent / "device" / "_bin" / "probe_invdist"
     assert exe.exists(), "build first: python -m utc_amd.build"
     say("case: probe_invdist (all floats 2^-96..inf)")
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)

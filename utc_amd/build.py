@@ -138,11 +138,17 @@ def build_oracle() -> None:
     _run([make, "-s", "-C", str(ROOT / "oracle")])
 
 
-def build_probes() -> list[Path]:
-    """Stand-alone gfx950 microbenchmarks (utc_amd/bench/*.hip) into utc_amd/lib/."""
+TEST_DEVICE = ROOT / "tests" / "device"
+
+
+def build_test_programs() -> list[Path]:
+    """Stand-alone gfx950 check programs the GPU tests run (tests/device/*.hip),
+    into tests/device/_bin/ (test infrastructure, not part of the library)."""
     outs = []
-    for src in sorted((PKG / "bench").glob("*.hip")):
-        exe = OUT / src.stem
+    out_dir = TEST_DEVICE / "_bin"
+    for src in sorted(TEST_DEVICE.glob("*.hip")):
+        out_dir.mkdir(parents=True, exist_ok=True)
+        exe = out_dir / src.stem
         _run([HIPCC, *HIP_FLAGS, "-o", str(exe), str(src)])
         outs.append(exe)
     return outs
@@ -154,7 +160,7 @@ def build_all() -> None:
         lib = build_library()
         f_oracle.result()
     build_host_programs()
-    build_probes()
+    build_test_programs()
     print(f"built {lib}")
 
 

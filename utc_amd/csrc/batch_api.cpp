@@ -154,6 +154,13 @@ namespace {
 
 void release_adj(utcnb_batch* b)
 {
+    // a captured graph holds the addresses of the buffers freed below: drop
+    // it, so the next mode 0 run captures again instead of replaying launches
+    // on freed memory
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    b->g_nsteps = -1;
     dfree(b->ckpt);
     dfree(b->segpos);
     dfree(b->wpos);

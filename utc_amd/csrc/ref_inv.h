@@ -20,7 +20,7 @@
 // 1/inf = 0 (an overflowed d2) instead of inf * 0; NaN stays NaN.
 //
 // The compiler's generic sequences rescale tiny inputs; from 2^-96 up they
-// are not needed (s and 1/s are normal).  utc_amd/bench/probe_invdist.hip
+// are not needed (s and 1/s are normal).  tests/device/probe_invdist.hip
 // checks this function against the compiler's correctly rounded sqrtf and
 // division for every float d2 in [2^-96, inf] and NaN on the device.
 #pragma once
@@ -71,7 +71,7 @@ __device__ __forceinline__ float ref_inv_dist(float d2)
 // only for extreme exponent gaps and handle special values.  Between 2^-600
 // and 2^600, g and 1/g stay hundreds of binades from every such threshold, so
 // the same operations in the same order give the same bits.
-// utc_amd/bench/probe_invdist.hip compares the two on the device.
+// tests/device/probe_invdist.hip compares the two on the device.
 constexpr double kRefInv64Min = 0x1p-600, kRefInv64Max = 0x1p600;
 
 // sqrt(x) for x in [kRefInv64Min, kRefInv64Max]: the generic sequence's

@@ -77,6 +77,8 @@ public:
     }
 
     bool failed() const { return m_rt && m_rt->failed(); }
+    // jobs some local worker has not taken yet (not in the reference)
+    size_t queuedJobs() const { return m_rt ? m_rt->queuedJobs() : 0; }
     std::string error() const { return m_rt ? m_rt->error() : std::string(); }
     T* getUserTaskObj() { return m_obj; }
     bool hasActiveLocalThread() const { return m_rt && !m_rt->finished(); }

@@ -8,12 +8,15 @@
 // finish() (also run by the destructor) appends a finish job and joins.  A GPU
 // task binds worker k to device k % device_count and gives it its own HIP
 // stream (default priority).  An exception escaping a job is reported on
-// stderr and recorded (failed()); it never blocks wait().
+// stderr and recorded (failed()); it never blocks wait().  A job leaves the
+// queue (with its bound arguments) once every worker has taken it, so a task
+// that runs many jobs keeps only the ones some worker has not reached.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 
 #include <condition_variable>
+#include <deque>
 #include <functional>
 #include <mutex>
 #include <string>
@@ -40,6 +43,8 @@ public:
     bool failed() const;
     std::string error() const;
     int numThreads() const { return m_nthreads; }
+    // jobs still queued (not yet taken by every worker)
+    size_t queuedJobs() const;
 
     // barrier among this task's workers (used by intra_Barrier)
     void barrier();
@@ -56,7 +61,14 @@ private:
 
     mutable std::mutex m_mu;
     std::condition_variable m_cv;
-    std::vector<std::pair<Job, std::function<void()>>> m_jobs;
+    struct Entry {
+        Job kind;
+        std::function<void()> fn;
+        int pending;  // workers that have not taken the job yet
+    };
+    std::deque<Entry> m_jobs;  // jobs [m_base, m_base + m_jobs.size())
+    size_t m_base = 0;
+    void push(Job kind, std::function<void()> fn);  // caller holds m_mu
     // wait latch
     int m_waitPending = 0;
     std::condition_variable m_waitCv;

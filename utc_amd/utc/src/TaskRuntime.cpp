@@ -38,20 +38,31 @@ TaskRuntime::~TaskRuntime()
         finish();
 }
 
+void TaskRuntime::push(Job kind, std::function<void()> fn)
+{
+    m_jobs.push_back(Entry{kind, std::move(fn), m_nthreads});
+}
+
 void TaskRuntime::enqueue(Job kind, std::function<void()> fn)
 {
     {
         std::lock_guard<std::mutex> lk(m_mu);
-        m_jobs.emplace_back(kind, std::move(fn));
+        push(kind, std::move(fn));
     }
     m_cv.notify_all();
+}
+
+size_t TaskRuntime::queuedJobs() const
+{
+    std::lock_guard<std::mutex> lk(m_mu);
+    return m_jobs.size();
 }
 
 void TaskRuntime::wait()
 {
     std::unique_lock<std::mutex> lk(m_mu);
     m_waitPending = m_nthreads;
-    m_jobs.emplace_back(Job::wait, std::function<void()>());
+    push(Job::wait, std::function<void()>());
     m_cv.notify_all();
     m_waitCv.wait(lk, [this] { return m_waitPending == 0; });
 }
@@ -114,9 +125,18 @@ void TaskRuntime::threadMain(int lrank)
     size_t idx = 0;
     for (;;) {
         std::unique_lock<std::mutex> lk(m_mu);
-        m_cv.wait(lk, [&] { return m_jobs.size() > idx; });
-        Job kind = m_jobs[idx].first;
-        std::function<void()> fn = m_jobs[idx].second;
+        m_cv.wait(lk, [&] { return m_base + m_jobs.size() > idx; });
+        Entry& e = m_jobs[idx - m_base];
+        const Job kind = e.kind;
+        // the last worker to take a job moves its function out; the queue then
+        // drops every leading job all workers have taken
+        std::function<void()> fn = e.pending == 1 ? std::move(e.fn) : e.fn;
+        if (--e.pending == 0) {
+            while (!m_jobs.empty() && m_jobs.front().pending == 0) {
+                m_jobs.pop_front();
+                ++m_base;
+            }
+        }
         lk.unlock();
         ++idx;
         if (kind == Job::finish)

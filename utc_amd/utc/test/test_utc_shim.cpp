@@ -1,6 +1,8 @@
 // test_utc_shim.cpp -- CPU self-test of the UTC task shim (no GPU needed).
 // Exit status 0 when every check passes; each failure prints its line.
+#include <algorithm>
 #include <atomic>
+#include <memory>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -48,6 +50,11 @@ public:
             log.push_back("run" + std::to_string(k));
             out[k] = value + k;
         }
+    }
+    void touch(std::shared_ptr<std::vector<char>> blob)
+    {
+        if (__localThreadId == 0)
+            value += (int)blob->size() - 1023;
     }
     void addOne()
     {
@@ -102,6 +109,28 @@ int main()
         CHECK(r->runs == 16);
         t.finish();
         t.finish();  // idempotent
+    }
+    {
+        // 10^5 exec jobs, each binding a 1 KiB argument: a job leaves the queue
+        // (and releases its arguments) once every worker has taken it, so
+        // memory stays bounded however many jobs a task runs
+        Task<Recorder> t(ProcList(3, 0));
+        t.init(0);
+        auto blob = std::make_shared<std::vector<char>>(1024);
+        size_t peak = 0;
+        for (int k = 0; k < 100000; ++k) {
+            t.exec(&Recorder::touch, blob);
+            if (k % 1000 == 999) {
+                t.wait();
+                peak = std::max(peak, t.queuedJobs());
+            }
+        }
+        t.wait();
+        CHECK(t.getUserTaskObj()->value == 100000);
+        CHECK(t.queuedJobs() == 0);
+        CHECK(peak == 0);
+        CHECK(blob.use_count() == 1);  // no job holds the bound argument any more
+        t.finish();
     }
     {
         // a task with no thread on this process: every call is a no-op
