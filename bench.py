@@ -507,6 +507,24 @@ def main():
         "avg_launch_ms": t_launch * 1e3,
     }
 
+    # ---- the targets of the workload, pass/fail ---------------------------------
+    def target(value, op, bound, what):
+        ok = None if value is None else (value >= bound if op == ">=" else value <= bound)
+        return {"value": value, "target": f"{op} {bound}", "pass": ok, "what": what}
+
+    adj_ratio = fwd_adj.get("ratio_vs_forward") if isinstance(fwd_adj, dict) else None
+    targets = {
+        "fwd_frac": target(achieved / PEAK_FP32, ">=", 0.6,
+                           "forward step, 20 FLOP x N^2 per step / step time / 157.3 TFLOP/s"),
+        "fwd_adj_ratio": target(adj_ratio, ">=", 0.4,
+                                f"forward+adjoint forward-equivalent rate / forward rate, N={n}, "
+                                f"{K} steps, checkpoint every 10"),
+        "final_pos_relerr": target(acc["mixed"]["final_pos_relerr"], "<=", 1e-6,
+                                   f"mixed path vs the FP64 path after {K} steps"),
+    }
+    for k, t in targets.items():
+        log(f"target {k}: {t['value']} {t['target']} -> {'pass' if t['pass'] else 'MISS'}")
+
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("BENCH_CPU_THREADS", min(16, os.cpu_count() or 1)))
@@ -535,9 +553,17 @@ def main():
         },
         "unequal_masses": unequal,
         "vs_plain_fp32": {
+            "fp32_path": "PREC_FP32: plain FP32 pair sums (uncompensated block partials, FP32 "
+                         "reduction of the partials), FP32 state",
+            "fp32_tile": results["fp32"]["tile"],
             "fp32_interactions_per_s": results["fp32"]["interactions_per_s"],
             "mixed_over_fp32": m["interactions_per_s"] / results["fp32"]["interactions_per_s"],
+            "fp32_accel_relerr": acc["fp32"]["accel_relerr"],
+            "mixed_accel_relerr": acc["mixed"]["accel_relerr"],
+            "fp32_final_pos_relerr": acc["fp32"]["final_pos_relerr"],
+            "mixed_final_pos_relerr": acc["mixed"]["final_pos_relerr"],
         },
+        "targets": targets,
         "variants": results,
         "accuracy_vs_fp64": acc,
         "plummer65536_vs_fp64": cfg2,

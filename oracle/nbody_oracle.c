@@ -444,27 +444,89 @@ static inline void ksum_add(ksum* k, double x)
 
 static inline double ksum_val(const ksum* k) { return k->s + k->c; }
 
+/* The compensated FP64 field on body i (three values into out3). */
+static void accel_row_f64(int n, const double* pos4, double eps2, int i, double* out3)
+{
+    const double* pi = pos4 + 4 * (size_t)i;
+    ksum ax = {0, 0}, ay = {0, 0}, az = {0, 0};
+    for (int j = 0; j < n; ++j) {
+        if (j == i)
+            continue;
+        const double* pj = pos4 + 4 * (size_t)j;
+        double rx = pj[0] - pi[0], ry = pj[1] - pi[1], rz = pj[2] - pi[2];
+        double d2 = rx * rx + ry * ry + rz * rz + eps2;
+        double inv = 1.0 / sqrt(d2);
+        double s = pj[3] * (inv * inv * inv);
+        ksum_add(&ax, rx * s);
+        ksum_add(&ay, ry * s);
+        ksum_add(&az, rz * s);
+    }
+    out3[0] = ksum_val(&ax);
+    out3[1] = ksum_val(&ay);
+    out3[2] = ksum_val(&az);
+}
+
 void orc_accel_f64(int n, const double* pos4, double eps2, double* acc3)
 {
-    for (int i = 0; i < n; ++i) {
-        const double* pi = pos4 + 4 * (size_t)i;
-        ksum ax = {0, 0}, ay = {0, 0}, az = {0, 0};
-        for (int j = 0; j < n; ++j) {
-            if (j == i)
-                continue;
-            const double* pj = pos4 + 4 * (size_t)j;
-            double rx = pj[0] - pi[0], ry = pj[1] - pi[1], rz = pj[2] - pi[2];
-            double d2 = rx * rx + ry * ry + rz * rz + eps2;
-            double inv = 1.0 / sqrt(d2);
-            double s = pj[3] * (inv * inv * inv);
-            ksum_add(&ax, rx * s);
-            ksum_add(&ay, ry * s);
-            ksum_add(&az, rz * s);
-        }
-        acc3[3 * (size_t)i + 0] = ksum_val(&ax);
-        acc3[3 * (size_t)i + 1] = ksum_val(&ay);
-        acc3[3 * (size_t)i + 2] = ksum_val(&az);
+    for (int i = 0; i < n; ++i)
+        accel_row_f64(n, pos4, eps2, i, acc3 + 3 * (size_t)i);
+}
+
+/* Rows rows[0..nrows) of orc_accel_f64, threaded with the reference partition
+ * rule over the row list (the first nrows%T workers take one row more). */
+typedef struct {
+    int n;
+    const double* pos4;
+    double eps2;
+    const int* rows;
+    int k0, k1;
+    double* acc3;
+    int cpu;
+} accel_rows_job;
+
+static void* accel_rows_worker(void* p)
+{
+    accel_rows_job* jb = (accel_rows_job*)p;
+    if (jb->cpu >= 0) {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        CPU_SET(jb->cpu, &set);
+        (void)pthread_setaffinity_np(pthread_self(), sizeof(set), &set); /* best effort */
     }
+    for (int k = jb->k0; k < jb->k1; ++k)
+        accel_row_f64(jb->n, jb->pos4, jb->eps2, jb->rows[k], jb->acc3 + 3 * (size_t)k);
+    return NULL;
+}
+
+int orc_accel_rows_f64(int n, const double* pos4, double eps2, int nrows, const int* rows,
+                       double* acc3, int nthreads)
+{
+    for (int k = 0; k < nrows; ++k)
+        if (rows[k] < 0 || rows[k] >= n)
+            return -1;
+    if (nthreads < 1)
+        nthreads = 1;
+    if (nthreads > nrows)
+        nthreads = nrows > 0 ? nrows : 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    accel_rows_job* jobs = (accel_rows_job*)malloc(sizeof(accel_rows_job) * (size_t)nthreads);
+    int per = nrows / nthreads, extra = nrows % nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        int local = t < extra ? per + 1 : per;
+        int start = t < extra ? t * (per + 1) : t * per + extra;
+        jobs[t] = (accel_rows_job){n, pos4, eps2, rows, start, start + local, acc3,
+                                   nthreads > 1 ? allowed_cpu(t) : -1};
+        if (nthreads > 1)
+            pthread_create(&th[t], NULL, accel_rows_worker, &jobs[t]);
+        else
+            accel_rows_worker(&jobs[t]);
+    }
+    if (nthreads > 1)
+        for (int t = 0; t < nthreads; ++t)
+            pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+    return 0;
 }
 
 void orc_accel_ld(int n, const double* pos4, double eps2, double* acc3)

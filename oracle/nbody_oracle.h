@@ -62,6 +62,10 @@ int orc_write_frames_f64(const char* path, const double* frames, int nframes, in
 void orc_accel_f64(int n, const double* pos4, double eps2, double* acc3);
 /* same, summed in long double with the pair in long double (tiny-N brute force) */
 void orc_accel_ld(int n, const double* pos4, double eps2, double* acc3);
+/* rows rows[0..nrows) of orc_accel_f64 (bit for bit) into acc3[3*nrows],
+ * threaded with the reference partition rule; -1 if a row is out of range */
+int orc_accel_rows_f64(int n, const double* pos4, double eps2, int nrows, const int* rows,
+                       double* acc3, int nthreads);
 /* kick=0: v=(v+a*m_i*dt)*damping (reference kick factor), kick=1: v=(v+a*dt)*damping;
  * then x += v*dt.  State in FP64. */
 void orc_step_f64(int n, double* pos4, double* vel4, double eps2, double dt,

@@ -48,6 +48,7 @@ def lib():
         o.orc_write_frames_f64.argtypes = [C.c_char_p, _vp, C.c_int, C.c_int, _f]
         o.orc_accel_f64.argtypes = [C.c_int, _vp, _d, _vp]
         o.orc_accel_ld.argtypes = [C.c_int, _vp, _d, _vp]
+        o.orc_accel_rows_f64.argtypes = [C.c_int, _vp, _d, C.c_int, _vp, _vp, C.c_int]
         o.orc_step_f64.argtypes = [C.c_int, _vp, _vp, _d, _d, _d, C.c_int]
         o.orc_energy_f64.argtypes = [C.c_int, _vp, _vp, _d, C.POINTER(_d), C.POINTER(_d), _vp]
         o.orc_loss_f64.argtypes = [C.c_int, C.c_int, _vp, _vp, _vp, _d, _d, _d, C.c_int]
@@ -135,6 +136,16 @@ def accel_f64(pos4, eps2):
     p = np.ascontiguousarray(pos4, np.float64)
     a = np.zeros((len(p), 3), np.float64)
     lib().orc_accel_f64(len(p), _ptr(p), _d(eps2), _ptr(a))
+    return a
+
+
+def accel_rows_f64(pos4, eps2, rows, nthreads=8):
+    """Rows `rows` of accel_f64 (bit for bit), threaded."""
+    p = np.ascontiguousarray(pos4, np.float64)
+    r = np.ascontiguousarray(rows, np.int32)
+    a = np.zeros((len(r), 3), np.float64)
+    if lib().orc_accel_rows_f64(len(p), _ptr(p), _d(eps2), len(r), _ptr(r), _ptr(a), nthreads) != 0:
+        raise ValueError("row index out of range")
     return a
 
 

@@ -230,19 +230,26 @@ def test_kernel_level_entry_point_f64_matches_oracle():
         assert np.array_equal(v.cpu().numpy(), ov), blocksize
 
 
-def test_fp32_cube4096_within_tolerance_of_reference():
-    """Plain fast FP32 (rsq, FMA) vs the reference CPU path on the N=4096 cube:
-    max |dx| <= 1e-6 and relative norm <= 1e-6."""
+@pytest.mark.parametrize("tile", ["default", "k2s8", "sym4r32"])
+def test_fp32_cube4096_within_tolerance_of_reference(tile):
+    """Plain fast FP32 (rsq, FMA, uncompensated FP32 sums, FP32 state) vs the
+    reference CPU path on the N=4096 cube: max |dx| <= 1e-6 and relative norm
+    <= 1e-6.  The default tile at this size (symmetric, plain FP32 block sums
+    and an FP32 reduction of the partials), a one-pass tile and another
+    symmetric one."""
     c = cube4096()
     pos = c["pos"].astype(np.float32)
     vel = c["vel"].astype(np.float32)
     eps2 = np.float32(c["eps"]) * np.float32(c["eps"])
     dt = np.float32(c["dt"])
     op, _ = O.run_f32(pos, vel, c["steps"], eps2, 1.0, dt, nthreads=8)
+    tid = 0 if tile == "default" else U.tiles().index(tile) + 1
     for flags in (0, U.FLAG_NO_NEWTON):
-        say(f"case: FP32 cube4096 flags={flags}")
+        say(f"case: FP32 cube4096 tile={tile} flags={flags}")
         with U.System(len(pos), U.PREC_FP32, float(eps2), float(dt), 1.0,
-                      kick=U.KICK_REFERENCE, flags=flags) as s:
+                      kick=U.KICK_REFERENCE, flags=flags, tile=tid) as s:
+            if tile == "default":
+                assert s.tile()[1] == "sym2r32"
             s.set_state(pos, vel)
             s.step(c["steps"])
             gp, _ = s.get_state(np.float32)
@@ -372,6 +379,45 @@ def test_full_size_accuracy_against_fp64(which):
     say(f"  final-position rel err vs FP64: mixed {errs[U.PREC_MIXED]:.3e}, fp32 {errs[U.PREC_FP32]:.3e}")
     assert errs[U.PREC_MIXED] <= 1e-6
     assert errs[U.PREC_FP32] <= 1e-6
+
+
+@pytest.mark.parametrize("which", ["plummer65536", "plummer131072", "plummer262144"])
+def test_full_size_fields_against_cpu_fp64_oracle(which):
+    """The yardstick of the full-size tests, pinned to the CPU: at N = 65536
+    (configuration 2), 131072 and 262144 (configuration 3) -- the three launch
+    shapes of the FP64 kernel (4, 8 and 16 source splits) -- the FP64 field of
+    512 sampled bodies against the CPU FP64 oracle (compensated sums over all N
+    sources), at the initial state and after the configuration's steps; and
+    the mixed (compensated) and plain FP32 fields of the same bodies against
+    it.  Tolerances: FP64 1e-14 relative, mixed 2e-6, plain FP32 4e-6."""
+    if which == "plummer131072":
+        c = plummer(131072, eps=0.002, dt=2.5e-4, steps=10)
+    else:
+        c = plummer65536() if which == "plummer65536" else plummer262144()
+    n, eps2 = len(c["pos"]), c["eps"] ** 2
+    rows = np.sort(np.random.default_rng(n).choice(n, 512, replace=False))
+    say(f"case: {which} sampled rows vs CPU FP64 oracle n={n}")
+    with U.System(n, U.PREC_FP64, eps2, c["dt"]) as s64:
+        s64.set_state(c["pos"], c["vel"])
+        a64 = s64.accel()
+        s64.step(c["steps"])
+        p_end, _ = s64.get_state()
+        a64_end = s64.accel()
+    for tag, pos, field in (("initial", c["pos"], a64), ("final", p_end, a64_end)):
+        ref = O.accel_rows_f64(pos, eps2, rows, nthreads=16)
+        err = relnorm(field[rows], ref)
+        say(f"  FP64 field ({tag} state) vs CPU oracle: rel {err:.3e}")
+        assert err <= 1e-14, tag
+        assert np.abs(field[rows] - ref).max() <= 1e-13 * np.abs(ref).max(), tag
+    ref = O.accel_rows_f64(c["pos"], eps2, rows, nthreads=16)
+    errs = {}
+    for prec in (U.PREC_MIXED, U.PREC_FP32):
+        with U.System(n, prec, eps2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+            s.set_state(c["pos"], c["vel"])
+            errs[prec] = relnorm(s.accel()[rows], ref)
+    say(f"  mixed field rel {errs[U.PREC_MIXED]:.3e}, plain FP32 field rel {errs[U.PREC_FP32]:.3e}")
+    assert errs[U.PREC_MIXED] <= 2e-6
+    assert errs[U.PREC_FP32] <= 4e-6
 
 
 def test_large_n_symmetric_default_matches_fp64():

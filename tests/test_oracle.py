@@ -101,6 +101,22 @@ def test_threaded_rows_equal_serial():
 
 
 # ---- FP64 physical-mode oracle ------------------------------------------------------
+def test_accel_rows_f64_equal_full_field():
+    """The threaded row-subset FP64 field (the yardstick of the full-size GPU
+    tests) equals the serial full field bit for bit, for any row list and
+    thread count; out-of-range rows are refused."""
+    rng = np.random.default_rng(11)
+    n = 777
+    pos = np.zeros((n, 4)); pos[:, :3] = rng.normal(size=(n, 3)); pos[:, 3] = rng.uniform(0.5, 2, n) / n
+    full = O.accel_f64(pos, 1e-4)
+    rows = rng.choice(n, 97, replace=False)
+    for t in (1, 3, 8, 200):
+        assert np.array_equal(O.accel_rows_f64(pos, 1e-4, rows, t), full[rows])
+    assert np.array_equal(O.accel_rows_f64(pos, 1e-4, np.arange(n), 5), full)
+    with pytest.raises(ValueError):
+        O.accel_rows_f64(pos, 1e-4, [0, n], 2)
+
+
 def test_accel_f64_against_long_double_brute_force():
     rng = np.random.default_rng(0)
     n = 200

@@ -72,7 +72,7 @@ int tile_default(int64_t n, int cu_count, size_t partial_budget)
 }
 
 template <class Epi>
-hipError_t launch_sym(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
+hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, int n, int n_pad, float eps2,
                       float* ipart, float* jpart, const Epi& epi, hipStream_t st);
 
 hipError_t launch_fp32_step(const StepArgs& a, const float4* pos_old, float4* pos_new,
@@ -80,8 +80,8 @@ hipError_t launch_fp32_step(const StepArgs& a, const float4* pos_old, float4* po
 {
     EpiFP32 epi{pos_old, pos_new, vel, (float)a.dt, (float)a.damping, a.kick};
     if (tile_get(a.tile).sym)
-        return launch_sym(tile_get(a.tile), a.newton, pos_old, a.n, a.n_pad, a.eps2f, a.ipart, a.jpart,
-                          epi, st);
+        return launch_sym(tile_get(a.tile), a.newton, false, pos_old, a.n, a.n_pad, a.eps2f, a.ipart,
+                          a.jpart, epi, st);
     if (a.newton)
         return launch_force<false, true>(a.tile, pos_old, a.n, a.n_pad, a.eps2f, epi, st);
     return launch_force<false, false>(a.tile, pos_old, a.n, a.n_pad, a.eps2f, epi, st);
@@ -92,8 +92,8 @@ hipError_t launch_mixed_step(const StepArgs& a, const float4* snap_old, float4* 
 {
     EpiMixed epi{pos, vel, snap_new, a.dt, a.damping, a.kick};
     if (tile_get(a.tile).sym)
-        return launch_sym(tile_get(a.tile), a.newton, snap_old, a.n, a.n_pad, a.eps2f, a.ipart, a.jpart,
-                          epi, st);
+        return launch_sym(tile_get(a.tile), a.newton, true, snap_old, a.n, a.n_pad, a.eps2f, a.ipart,
+                          a.jpart, epi, st);
     if (a.newton)
         return launch_force<true, true>(a.tile, snap_old, a.n, a.n_pad, a.eps2f, epi, st);
     return launch_force<true, false>(a.tile, snap_old, a.n, a.n_pad, a.eps2f, epi, st);
@@ -104,8 +104,8 @@ hipError_t launch_fp32_accel(const StepArgs& a, const float4* src, bool compensa
 {
     EpiAccel epi{acc3};
     if (tile_get(a.tile).sym)
-        return launch_sym(tile_get(a.tile), a.newton, src, a.n, a.n_pad, a.eps2f, a.ipart, a.jpart, epi,
-                          st);
+        return launch_sym(tile_get(a.tile), a.newton, compensated, src, a.n, a.n_pad, a.eps2f, a.ipart,
+                          a.jpart, epi, st);
     if (compensated) {
         if (a.newton)
             return launch_force<true, true>(a.tile, src, a.n, a.n_pad, a.eps2f, epi, st);

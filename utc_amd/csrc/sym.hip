@@ -49,12 +49,24 @@ __device__ __forceinline__ float wave_ror1(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(K <= 2 ? x : 0, x, 0x13C /* wave_ror:1 */, 0xF, 0xF, false));
 }
 
+// Adds a block partial p into the running target sum: Kahan-compensated
+// (COMP) or plain FP32.
+template <bool COMP>
+__device__ __forceinline__ void block_add(float& a, float& c, float p)
+{
+    if constexpr (COMP)
+        kahan_add(a, c, p);
+    else
+        a += p;
+}
+
 // A block of 64 sources (one per lane, s) against the K targets of each lane,
-// both sides: the target-side block partials are Kahan-added into (a, c), the
-// reaction sums of the lane's source come back in g after 64 rotations.
+// both sides: the target-side block partials are added into (a, c) by
+// block_add<COMP>, the reaction sums of the lane's source come back in g after
+// 64 rotations.
 // UNI: every target mass mi[k] and source mass equals m, so the pair terms
 // drop the mass factors and the block sums are scaled by m at the end.
-template <int K, bool NEWTON, bool UNI>
+template <int K, bool NEWTON, bool COMP, bool UNI>
 __device__ __forceinline__ void sym_block(float4 s, const float (&xi)[K], const float (&yi)[K],
                                           const float (&zi)[K], const float (&mi)[K], float m, float eps2,
                                           float (&ax)[K], float (&ay)[K], float (&az)[K], float (&cx)[K],
@@ -92,9 +104,9 @@ __device__ __forceinline__ void sym_block(float4 s, const float (&xi)[K], const 
         if constexpr (UNI) {
             px[k] *= m; py[k] *= m; pz[k] *= m;
         }
-        kahan_add(ax[k], cx[k], px[k]);
-        kahan_add(ay[k], cy[k], py[k]);
-        kahan_add(az[k], cz[k], pz[k]);
+        block_add<COMP>(ax[k], cx[k], px[k]);
+        block_add<COMP>(ay[k], cy[k], py[k]);
+        block_add<COMP>(az[k], cz[k], pz[k]);
     }
     if constexpr (UNI) {
         gx *= m; gy *= m; gz *= m;
@@ -107,7 +119,7 @@ __device__ __forceinline__ void sym_block(float4 s, const float (&xi)[K], const 
 // adds r = 0); slots k > KB take these sources when their own block is the
 // source.  The reaction sums come home to slot KB after 64 rotations.  UNI as
 // in sym_block.
-template <int K, bool NEWTON, bool UNI, int KB>
+template <int K, bool NEWTON, bool COMP, bool UNI, int KB>
 __device__ __forceinline__ void sym_own(float4 s, const float (&xi)[K], const float (&yi)[K],
                                         const float (&zi)[K], const float (&mi)[K], float m, float eps2,
                                         float (&ax)[K], float (&ay)[K], float (&az)[K], float (&cx)[K],
@@ -155,22 +167,22 @@ __device__ __forceinline__ void sym_own(float4 s, const float (&xi)[K], const fl
         if constexpr (UNI) {
             px[k] *= m; py[k] *= m; pz[k] *= m;
         }
-        kahan_add(ax[k], cx[k], px[k]);
-        kahan_add(ay[k], cy[k], py[k]);
-        kahan_add(az[k], cz[k], pz[k]);
+        block_add<COMP>(ax[k], cx[k], px[k]);
+        block_add<COMP>(ay[k], cy[k], py[k]);
+        block_add<COMP>(az[k], cz[k], pz[k]);
     }
     if constexpr (KB > 0) {
         if constexpr (UNI) {
             gx *= m; gy *= m; gz *= m;
         }
-        kahan_add(ax[KB], cx[KB], gx);
-        kahan_add(ay[KB], cy[KB], gy);
-        kahan_add(az[KB], cz[KB], gz);
+        block_add<COMP>(ax[KB], cx[KB], gx);
+        block_add<COMP>(ay[KB], cy[KB], gy);
+        block_add<COMP>(az[KB], cz[KB], gz);
     }
 }
 
 // sym_own for the wave-uniform block index kb, 0 <= kb < K.
-template <int K, bool NEWTON, int KB = 0>
+template <int K, bool NEWTON, bool COMP, int KB = 0>
 __device__ __forceinline__ void sym_own_at(int kb, bool uni, float4 s, const float (&xi)[K], const float (&yi)[K],
                                            const float (&zi)[K], const float (&mi)[K], float m, float eps2,
                                            float (&ax)[K], float (&ay)[K], float (&az)[K], float (&cx)[K],
@@ -178,18 +190,19 @@ __device__ __forceinline__ void sym_own_at(int kb, bool uni, float4 s, const flo
 {
     if (kb == KB) {
         if (uni)
-            sym_own<K, NEWTON, true, KB>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
+            sym_own<K, NEWTON, COMP, true, KB>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
         else
-            sym_own<K, NEWTON, false, KB>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
+            sym_own<K, NEWTON, COMP, false, KB>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
     } else if constexpr (KB + 1 < K) {
-        sym_own_at<K, NEWTON, KB + 1>(kb, uni, s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
+        sym_own_at<K, NEWTON, COMP, KB + 1>(kb, uni, s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
     }
 }
 
 // The work of wave r of wave block w of one system: src = the system's
 // padded bodies (NW wave blocks of 64K); ipart: [R][n_pad][3] target-side
-// sums; jpart: [NW][NW/2][64K][3] reactions.
-template <int K, bool NEWTON>
+// sums; jpart: [NW][NW/2][64K][3] reactions.  COMP: the target-side block
+// partials are Kahan-added (the compensated paths); otherwise plain FP32 sums.
+template <int K, bool NEWTON, bool COMP>
 __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW, int R, int w, int r,
                                          float eps2, float* __restrict__ ipart,
                                          float* __restrict__ jpart, int lane)
@@ -220,7 +233,7 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
                 // own block kb: two-sided against the lower target slots
                 const float4 s = src[w * 64 * K + kb * 64 + lane];
                 const bool uni = __ballot(tuni && s.w == m0) == ~0ull;
-                sym_own_at<K, NEWTON>(kb, uni, s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz);
+                sym_own_at<K, NEWTON, COMP>(kb, uni, s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz);
                 continue;
             }
         }
@@ -249,9 +262,9 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                kahan_add(ax[k], cx[k], px[k]);
-                kahan_add(ay[k], cy[k], py[k]);
-                kahan_add(az[k], cz[k], pz[k]);
+                block_add<COMP>(ax[k], cx[k], px[k]);
+                block_add<COMP>(ay[k], cy[k], py[k]);
+                block_add<COMP>(az[k], cz[k], pz[k]);
             }
             continue;
         }
@@ -264,9 +277,9 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
         const float4 s = src[v * 64 * K + kb * 64 + lane];
         float gx, gy, gz;
         if (__ballot(tuni && s.w == m0) == ~0ull)
-            sym_block<K, NEWTON, true>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
+            sym_block<K, NEWTON, COMP, true>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
         else
-            sym_block<K, NEWTON, false>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
+            sym_block<K, NEWTON, COMP, false>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
         const size_t o = (((size_t)w * half + d) * (64 * K) + kb * 64 + lane) * 3;
         jpart[o + 0] = gx;
         jpart[o + 1] = gy;
@@ -282,7 +295,7 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
 }
 
 // One system: wave gw = w * R + r.
-template <int K, bool NEWTON>
+template <int K, bool NEWTON, bool COMP>
 __global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict__ src, int NW, int R,
                                                         float eps2, float* __restrict__ ipart,
                                                         float* __restrict__ jpart)
@@ -290,7 +303,7 @@ __global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict
     const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (gw >= NW * R)
         return;  // whole waves only
-    sym_wave<K, NEWTON>(src, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
+    sym_wave<K, NEWTON, COMP>(src, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
 }
 
 // Batched systems: each wave runs items (system, wave block, split), a system
@@ -308,7 +321,7 @@ __global__ __launch_bounds__(256) void batch_sym_kernel(BatchSymGeom g, int nwav
     for (int it = batch_next_item(g, gw, pos, lane); it >= 0; it = batch_next_item(g, gw, pos, lane)) {
         const int4 item = g.items[it];  // (system, wave block, split, splits)
         const int s = item.x;
-        sym_wave<K, false>(snap + g.off[s], g.nw[s], item.w, item.y, item.z, eps2, ipart + g.ipo[s],
+        sym_wave<K, false, true>(snap + g.off[s], g.nw[s], item.w, item.y, item.z, eps2, ipart + g.ipo[s],
                            jpart + g.jpo[s], lane);
     }
 }
@@ -329,15 +342,23 @@ __device__ __forceinline__ void sym_epilogue(const Epi& epi, int i, const double
     epi(i, h0, h1, h2, (float)((double)h0 - s[0]), (float)((double)h1 - s[1]), (float)((double)h2 - s[2]));
 }
 
-// Body b's partials summed in FP64 by T lanes (sym_reduce.h), then the epilogue.
-template <int T, class Epi>
+// Plain FP32 sums go to the epilogue as they are (no compensation term).
+template <class Epi>
+__device__ __forceinline__ void sym_epilogue(const Epi& epi, int i, const float (&s)[3])
+{
+    epi(i, s[0], s[1], s[2], 0.f, 0.f, 0.f);
+}
+
+// Body b's partials summed by T lanes (sym_reduce.h) in FP64 (Acc = double,
+// the compensated paths) or plain FP32 (Acc = float), then the epilogue.
+template <int T, typename Acc, class Epi>
 __global__ __launch_bounds__(256) void sym_reduce_kernel(const float* __restrict__ ipart,
                                                          const float* __restrict__ jpart, int n, int NW,
                                                          int R, int K, Epi epi)
 {
     int b, t;
     sym_reduce_lane<T>(b, t);
-    double s[3];
+    Acc s[3];
     sym_split_sum<T>((const Part3*)ipart, (const Part3*)jpart, b < n ? b : n - 1, NW, R, K, t, s);
     if (b < n && t == 0)
         sym_epilogue(epi, b, s);
@@ -369,8 +390,35 @@ size_t sym_jpart_floats(int n_pad, int K)
     return (size_t)NW * (NW / 2) * 64 * K * 3;
 }
 
+template <int K, bool COMP>
+static void launch_sym_pairs(bool newton, dim3 grid, const float4* src, int NW, int R, float eps2, float* ipart,
+                             float* jpart, hipStream_t st)
+{
+    if (newton)
+        hipLaunchKernelGGL((sym_force_kernel<K, true, COMP>), grid, dim3(256), 0, st, src, NW, R, eps2, ipart, jpart);
+    else
+        hipLaunchKernelGGL((sym_force_kernel<K, false, COMP>), grid, dim3(256), 0, st, src, NW, R, eps2, ipart, jpart);
+}
+
+template <typename Acc, class Epi>
+static void launch_sym_reduce(int T, const float* ipart, const float* jpart, int n, int NW, int R, int K,
+                              const Epi& epi, hipStream_t st)
+{
+    const dim3 rgrid((unsigned)(((long long)n * T + 255) / 256));
+    switch (T) {
+    case 1: hipLaunchKernelGGL((sym_reduce_kernel<1, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    case 2: hipLaunchKernelGGL((sym_reduce_kernel<2, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    case 4: hipLaunchKernelGGL((sym_reduce_kernel<4, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    case 8: hipLaunchKernelGGL((sym_reduce_kernel<8, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    default: hipLaunchKernelGGL((sym_reduce_kernel<16, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    }
+}
+
+// comp: Kahan-added block partials and an FP64 reduction (the MIXED step and
+// the field of a MIXED handle); otherwise plain FP32 sums throughout
+// (PREC_FP32).
 template <class Epi>
-hipError_t launch_sym(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
+hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, int n, int n_pad, float eps2,
                       float* ipart, float* jpart, const Epi& epi, hipStream_t st)
 {
     const int K = t.k, R = t.s;
@@ -378,37 +426,26 @@ hipError_t launch_sym(const Tile& t, bool newton, const float4* src, int n, int 
         return hipErrorInvalidValue;
     const int NW = n_pad / (64 * K);
     const int waves = NW * R;
-    const dim3 grid((waves + 3) / 4), block(256);
-    if (K == 2) {
-        if (newton)
-            hipLaunchKernelGGL((sym_force_kernel<2, true>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
-        else
-            hipLaunchKernelGGL((sym_force_kernel<2, false>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
-    } else if (K == 4) {
-        if (newton)
-            hipLaunchKernelGGL((sym_force_kernel<4, true>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
-        else
-            hipLaunchKernelGGL((sym_force_kernel<4, false>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
-    } else if (K == 8) {
-        if (newton)
-            hipLaunchKernelGGL((sym_force_kernel<8, true>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
-        else
-            hipLaunchKernelGGL((sym_force_kernel<8, false>), grid, block, 0, st, src, NW, R, eps2, ipart, jpart);
-    } else {
+    const dim3 grid((waves + 3) / 4);
+    if (K == 2)
+        comp ? launch_sym_pairs<2, true>(newton, grid, src, NW, R, eps2, ipart, jpart, st)
+             : launch_sym_pairs<2, false>(newton, grid, src, NW, R, eps2, ipart, jpart, st);
+    else if (K == 4)
+        comp ? launch_sym_pairs<4, true>(newton, grid, src, NW, R, eps2, ipart, jpart, st)
+             : launch_sym_pairs<4, false>(newton, grid, src, NW, R, eps2, ipart, jpart, st);
+    else if (K == 8)
+        comp ? launch_sym_pairs<8, true>(newton, grid, src, NW, R, eps2, ipart, jpart, st)
+             : launch_sym_pairs<8, false>(newton, grid, src, NW, R, eps2, ipart, jpart, st);
+    else
         return hipErrorInvalidValue;
-    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
     const int T = sym_reduce_lanes(n);
-    const dim3 rgrid((unsigned)(((long long)n * T + 255) / 256));
-    switch (T) {
-    case 1: hipLaunchKernelGGL((sym_reduce_kernel<1, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
-    case 2: hipLaunchKernelGGL((sym_reduce_kernel<2, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
-    case 4: hipLaunchKernelGGL((sym_reduce_kernel<4, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
-    case 8: hipLaunchKernelGGL((sym_reduce_kernel<8, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
-    default: hipLaunchKernelGGL((sym_reduce_kernel<16, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
-    }
+    if (comp)
+        launch_sym_reduce<double>(T, ipart, jpart, n, NW, R, K, epi, st);
+    else
+        launch_sym_reduce<float>(T, ipart, jpart, n, NW, R, K, epi, st);
     return hipGetLastError();
 }
 
@@ -443,11 +480,11 @@ hipError_t launch_batch_sym_step(const BatchSymGeom& g, int nwaves, int total, c
     return hipGetLastError();
 }
 
-template hipError_t launch_sym<EpiFP32>(const Tile&, bool, const float4*, int, int, float, float*, float*,
+template hipError_t launch_sym<EpiFP32>(const Tile&, bool, bool, const float4*, int, int, float, float*, float*,
                                         const EpiFP32&, hipStream_t);
-template hipError_t launch_sym<EpiMixed>(const Tile&, bool, const float4*, int, int, float, float*, float*,
+template hipError_t launch_sym<EpiMixed>(const Tile&, bool, bool, const float4*, int, int, float, float*, float*,
                                          const EpiMixed&, hipStream_t);
-template hipError_t launch_sym<EpiAccel>(const Tile&, bool, const float4*, int, int, float, float*, float*,
+template hipError_t launch_sym<EpiAccel>(const Tile&, bool, bool, const float4*, int, int, float, float*, float*,
                                          const EpiAccel&, hipStream_t);
 
 }  // namespace utcnb

@@ -6,7 +6,7 @@
 // of n_pad records), q = R + d the reaction from the wave block d + 1 places
 // below (jpart), absent for the block half way round when the upper half
 // skipped it.  T lanes share a body: lane t adds records t, t + T, t + 2T, ...
-// in FP64 in that order, then the T partial sums are combined by a butterfly
+// in that order (in FP64 on the compensated paths, in FP32 on the plain one), then the T partial sums are combined by a butterfly
 // of __shfl_xor over fixed lane offsets.  The order depends only on (T, R,
 // NW), so results are bitwise repeatable; with T = 1 it is the plain
 // sequential order.  Lane layout in a wave: lane = t * (64/T) + j for the
@@ -21,7 +21,8 @@ struct Part3 {
     float x, y, z;
 };
 
-__device__ __forceinline__ void part_add(double (&s)[3], const Part3& p)
+template <typename Acc>
+__device__ __forceinline__ void part_add(Acc (&s)[3], const Part3& p)
 {
     s[0] += p.x; s[1] += p.y; s[2] += p.z;
 }
@@ -53,15 +54,15 @@ __device__ __forceinline__ void sym_reduce_lane(int& body, int& t)
 
 // All 64 lanes of the wave must call this (it shuffles); b must be a valid
 // body of the system for every lane.
-template <int T, typename P, int C>
+template <int T, typename P, typename Acc, int C>
 __device__ __forceinline__ void sym_split_sum(const P* __restrict__ ipart, const P* __restrict__ jpart, int b,
-                                              int NW, int R, int K, int t, double (&s)[C])
+                                              int NW, int R, int K, int t, Acc (&s)[C])
 {
     const int n_pad = NW * 64 * K;
     const int half = NW / 2, v = b / (64 * K), loc = b - v * (64 * K);
 #pragma unroll
     for (int c = 0; c < C; ++c)
-        s[c] = 0.0;
+        s[c] = Acc(0);
     const int Q = R + half;
 #pragma unroll 4
     for (int q = t; q < Q; q += T) {
