@@ -132,6 +132,24 @@ int utcnb_sync(utcnb_system* sys);
 /* Acceleration field of the current state, as the system's force kernel
  * computes it (FP32 hi+lo for MIXED), written as FP64 acc3[3n] (host). */
 int utcnb_accel(utcnb_system* sys, double* acc3);
+/*
+ * The field of utcnb_accel for n bodies given on the device, with no handle
+ * and no device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  pos4: device FP32 [4n] (x,y,z,m);
+ * acc3: device FP64 [3n]; asynchronous on `stream` (a hipStream_t, NULL =
+ * the null stream).  precision FP32 (plain sums) or MIXED (compensated), tile
+ * and flags (UTCNB_FLAG_NO_NEWTON) as in utcnb_config; the result is bitwise
+ * the utcnb_accel field of a handle of that precision, tile and flags whose
+ * FP32 source positions equal pos4.  Work memory (padded sources, partial
+ * sums) comes from a 64 MiB device array that the library holds on each
+ * device it runs on until the process ends; sizes whose work memory exceeds
+ * it are refused with UTCNB_EINVAL (with the default tile, n up to about
+ * 32768 on the symmetric tiles and about 4M on the one-pass ones).  Calls on
+ * one device share that array: calls that overlap on one device race.
+ */
+int utcnb_accel_noalloc(const float* pos4, int n, double eps2, int precision, int tile, int flags,
+                        double* acc3, void* stream);
 /* FP64 diagnostics of the current state: kinetic 1/2 sum m|v|^2, potential
  * -sum_{i<j} m_i m_j / sqrt(r^2+eps2) and momentum sum m v (host outputs). */
 int utcnb_diagnostics(utcnb_system* sys, double* kinetic, double* potential,

@@ -34,7 +34,7 @@ EXPORTS = [
     "utcnb_batch_destroy", "utcnb_batch_set_state", "utcnb_batch_get_state",
     "utcnb_batch_run", "utcnb_batch_get_tile", "utcnb_autotune", "utcnb_tile_count", "utcnb_tile_name",
     "utcnb_rng_seed", "utcnb_rng_next", "utcnb_splitmix64", "utcnb_generate",
-    "utcnb_last_error", "utcnb_version", "utcnb_device_info",
+    "utcnb_last_error", "utcnb_version", "utcnb_device_info", "utcnb_accel_noalloc",
 ]
 
 
@@ -94,6 +94,7 @@ def _proto(lib):
         "utcnb_last_error": (C.c_char_p, []),
         "utcnb_version": (C.c_char_p, []),
         "utcnb_device_info": (C.c_int, [C.c_char_p, C.c_int, P(C.c_int), P(C.c_int)]),
+        "utcnb_accel_noalloc": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):

@@ -61,6 +61,7 @@ LIB_SOURCES = [
     "capi.cpp",
     "gen.cpp",
     "autotune.cpp",
+    "noalloc.hip",
 ]
 
 
