@@ -294,6 +294,59 @@ def fwd_adjoint(U, c, n, K, fwd_rate, ckpt=10):
     return out
 
 
+def slice_per_gpu(U, c, n, K, shares=8):
+    """One GPU's share of the headline system sharded over `shares` GPUs by
+    targets (the reference's multi-GPU scheme,
+    benchapps/nbody/utc-snode-mgpu/mgpu/body_task_mgpu.cu:41-49 and
+    bodysystem_kernel.cu:88-140): share 0 (n/shares targets) against all n
+    sources, mixed precision, one-pass tiles (the symmetric ones pair bodies of
+    the whole system).  Every one-pass tile is timed; the default one and the
+    fastest are reported."""
+    start, cnt = U.partition(n, shares)[0]
+    pairs = float(cnt) * float(n)
+    names = U.tiles()
+    sweep = {}
+    for tid, name in enumerate(names, start=1):
+        if name.startswith("sym"):
+            continue
+        with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON, tile=tid) as s:
+            s.set_state(c["pos"], c["vel"])
+            for _ in range(2):  # warm-up
+                s.step_slice(start, cnt)
+            s.sync()
+            t0 = time.perf_counter()
+            for _ in range(K):
+                s.step_slice(start, cnt)
+                s.commit_step()
+            s.sync()
+            sweep[name] = (time.perf_counter() - t0) * 1e3 / K
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+        default = s.slice_tile(cnt)[1]
+        s.set_state(c["pos"], c["vel"])
+        s.step_slice(start, cnt)
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            s.step_slice(start, cnt)
+            s.commit_step()
+        s.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / K
+        kms, _ = s.last_kernel_ms()
+    best = min(sweep, key=sweep.get)
+    out = {
+        "shape": f"{cnt} targets x {n} sources (share 0 of {shares})", "targets": cnt, "sources": n,
+        "default_tile": default, "ms_per_step": ms, "kernel_ms_last_step": kms,
+        "interactions_per_s": pairs / (ms * 1e-3),
+        "fraction_of_fp32_peak": FLOP_PER_PAIR * pairs / (ms * 1e-3) / PEAK_FP32,
+        "best_tile": best, "best_ms_per_step": sweep[best],
+        "best_fraction_of_fp32_peak": FLOP_PER_PAIR * pairs / (sweep[best] * 1e-3) / PEAK_FP32,
+        "tile_sweep_ms": sweep,
+    }
+    log(f"slice {cnt}x{n} [{default}]: {ms:.3f} ms/step, {out['interactions_per_s']:.4e} interactions/s, "
+        f"{out['fraction_of_fp32_peak']:.3f} of peak; best {best} {sweep[best]:.3f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -305,7 +358,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=262144,
                     help="rows of the CPU baseline's force evaluation (default: all, ~10 s on 16 threads)")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched", "reference"], default=None,
+    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched", "reference", "slice"], default=None,
                     help="time one path only and print its line (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 256 MiB device copy (to scale HBM counters) and exit")
@@ -355,6 +408,11 @@ def main():
         print(json.dumps({"metric": METRIC, "value": r["interactions_per_s"], "unit": "interactions/s",
                           "n_gpus": 1, "steps": K, "only": "reference", "reference_arithmetic": r}),
               flush=True)
+        return
+    if args.only == "slice":
+        r = slice_per_gpu(U, c, n, K)
+        print(json.dumps({"metric": METRIC, "value": r["interactions_per_s"], "unit": "interactions/s",
+                          "n_gpus": 1, "steps": K, "only": "slice", "slice_per_gpu": r}), flush=True)
         return
     if args.only == "adjoint":
         r = fwd_adjoint(U, c, n, K, None)
@@ -471,6 +529,14 @@ def main():
             cfg4 = {"error": str(e)}
             log(f"N=16384 adjoint leg failed: {e}")
 
+    # ---- one GPU's share of the system sharded over 8 GPUs ----------------------------
+    slc = None
+    try:
+        slc = slice_per_gpu(U, c, n, K)
+    except Exception as e:  # reported, never hidden
+        slc = {"error": str(e)}
+        log(f"slice leg failed: {e}")
+
     # ---- 256 batched systems -------------------------------------------------------
     batch = None
     if not args.no_batch:
@@ -572,6 +638,7 @@ def main():
         "cube_logm16384_fwd_adjoint": cfg4,
         "reference_arithmetic": ref_arith,
         "batched": batch,
+        "slice_per_gpu": slc,
         "fraction_of_fp32_peak": achieved / PEAK_FP32,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -81,6 +81,26 @@ int utcnb_integrate_f64(const double* pos_old, double* pos_new, double* vel, int
                         double deltaTime, double softeningSquared, double damping,
                         int blocksize, void* stream);
 
+/*
+ * The multi-GPU form of the step above, for one GPU's share of the bodies:
+ * targets [start, start + count) against all numBodies sources of pos_old
+ * (T[4*numBodies]); the slice's new positions go to pos_new[4*count] and its
+ * velocities vel[4*count] (slice-local arrays, as the reference's per-GPU
+ * buffers) are updated in place.  Replaces the launch of the multi-GPU
+ * _integrateNBodySystem_kernel<T> (start_bodyIndex, local_numBodies) at
+ * benchapps/nbody/utc-snode-mgpu/mgpu/body_task_mgpu.cu:104-131 (kernel:
+ * benchapps/nbody/utc-snode-mgpu/mgpu/bodysystem_kernel.cu:88-140), with the
+ * same argument meaning; the reference's arithmetic bit for bit.  The caller
+ * partitions the bodies (the reference: the first numBodies % G shares take
+ * one body more, body_task_mgpu.cu:41-49) and gathers the slices.
+ */
+int utcnb_integrate_slice_f32(const float* pos_old, float* pos_new, float* vel, int numBodies, int start,
+                              int count, float deltaTime, float softeningSquared, float damping, int blocksize,
+                              void* stream);
+int utcnb_integrate_slice_f64(const double* pos_old, double* pos_new, double* vel, int numBodies, int start,
+                              int count, double deltaTime, double softeningSquared, double damping,
+                              int blocksize, void* stream);
+
 /* ---- system handle ------------------------------------------------------ */
 typedef struct utcnb_system utcnb_system;
 
@@ -128,6 +148,41 @@ int utcnb_get_state(utcnb_system* sys, void* pos4, void* vel4, int dtype, int wh
 int utcnb_step(utcnb_system* sys, int nsteps);
 /* Wait for the system's stream. */
 int utcnb_sync(utcnb_system* sys);
+
+/*
+ * One GPU's share of a step of a system sharded over targets (the reference's
+ * BodySystemMGPU<T>::runImpl step, benchapps/nbody/utc-snode-mgpu/mgpu/body_task_mgpu.cu:98-145):
+ * bodies [start, start + count) against the current positions of all n
+ * bodies; the slice's new positions go to the next position buffer, its
+ * velocities (and, for MIXED, its FP64 state) are updated in place.  The next
+ * buffer is not made current: the caller fills the other slices of it (its own
+ * further utcnb_step_slice calls, or the peers' slices received into
+ * utcnb_position_buffer(sys, 1, ...) by an all-gather), then calls
+ * utcnb_commit_step.  FP32 and MIXED run a one-pass tile (utcnb_slice_tile):
+ * the configured tile when it is one, else a default by count; REFERENCE and
+ * FP64 run their only kernels.  Per body the arithmetic does not depend on the
+ * slicing: slices covering [0, n) then a commit give bitwise the same state as
+ * utcnb_step(sys, 1) with that one-pass tile.  Asynchronous on the handle's
+ * stream; utcnb_last_kernel_ms times the slice.
+ */
+int utcnb_step_slice(utcnb_system* sys, int start, int count);
+/* Make the next position buffer current (after the slices of a step). */
+int utcnb_commit_step(utcnb_system* sys);
+/*
+ * Device address of the current (which = 0) or next (which = 1) position
+ * buffer: n records of 4 values (x, y, z, m) of *dtype (UTCNB_F32 for FP32,
+ * MIXED and REFERENCE/F32 -- the FP32 source snapshot for MIXED -- UTCNB_F64
+ * otherwise), the array an all-gather of slices writes.  No reference
+ * counterpart (the reference gathers slices through host memory,
+ * body_task_mgpu.cu:109,134-140).
+ */
+int utcnb_position_buffer(utcnb_system* sys, int which, void** pos4, int* dtype);
+/* Field of bodies [start, start + count) as utcnb_step_slice computes it:
+ * acc3 host FP64 [3 * count].  Synchronizes the stream. */
+int utcnb_accel_slice(utcnb_system* sys, int start, int count, double* acc3);
+/* Tile id utcnb_step_slice / utcnb_accel_slice run for `count` targets (0 for
+ * REFERENCE and FP64).  No reference counterpart. */
+int utcnb_slice_tile(const utcnb_system* sys, int count, int* tile);
 
 /* Acceleration field of the current state, as the system's force kernel
  * computes it (FP32 hi+lo for MIXED), written as FP64 acc3[3n] (host). */

@@ -78,9 +78,9 @@ void orc_gravitation_f32(int n, const float* pos4, float eps2, float* force3)
     grav_rows_f32(n, pos4, eps2, 0, n, force3);
 }
 
-void orc_gravitation_f64(int n, const double* pos4, double eps2, double* force3)
+static void grav_rows_f64(int n, const double* pos4, double eps2, int i0, int i1, double* force3)
 {
-    for (int i = 0; i < n; ++i) {
+    for (int i = i0; i < i1; ++i) {
         double acc[3] = {0.0, 0.0, 0.0};
         const double* pi = pos4 + 4 * (size_t)i;
         for (int j = 0; j < n; ++j)
@@ -89,6 +89,11 @@ void orc_gravitation_f64(int n, const double* pos4, double eps2, double* force3)
         force3[3 * (size_t)i + 1] = acc[1];
         force3[3 * (size_t)i + 2] = acc[2];
     }
+}
+
+void orc_gravitation_f64(int n, const double* pos4, double eps2, double* force3)
+{
+    grav_rows_f64(n, pos4, eps2, 0, n, force3);
 }
 
 /* Threaded rows.  Body partition follows
@@ -218,18 +223,62 @@ int orc_run_f32(int n, int steps, float* pos4, float* vel4, float eps2, float da
     return 0;
 }
 
-int orc_run_f64(int n, int steps, double* pos4, double* vel4, double eps2, double damping,
-                double dt)
+/* FP64 rows [i0, i1) of orc_gravitation_f64 on worker threads (the same
+ * partition rule as orc_gravitation_rows_f32); every row's sum is the serial
+ * one, so the result does not depend on the thread count. */
+typedef struct {
+    int n, i0, i1;
+    const double* pos4;
+    double eps2;
+    double* force3;
+} rows64_job;
+
+static void* rows64_worker(void* p)
+{
+    rows64_job* jb = (rows64_job*)p;
+    grav_rows_f64(jb->n, jb->pos4, jb->eps2, jb->i0, jb->i1, jb->force3);
+    return NULL;
+}
+
+static void grav_f64_threaded(int n, const double* pos4, double eps2, double* force3, int nthreads)
+{
+    if (nthreads <= 1 || n < nthreads) {
+        grav_rows_f64(n, pos4, eps2, 0, n, force3);
+        return;
+    }
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    rows64_job* jobs = (rows64_job*)malloc(sizeof(rows64_job) * (size_t)nthreads);
+    int per = n / nthreads, extra = n % nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        int local = t < extra ? per + 1 : per;
+        int start = t < extra ? t * (per + 1) : t * per + extra;
+        jobs[t] = (rows64_job){n, start, start + local, pos4, eps2, force3};
+        pthread_create(&th[t], NULL, rows64_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; ++t)
+        pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+}
+
+int orc_run_f64_mt(int n, int steps, double* pos4, double* vel4, double eps2, double damping,
+                   double dt, int nthreads)
 {
     double* f = (double*)malloc(sizeof(double) * 3 * (size_t)n);
     if (!f)
         return -1;
     for (int s = 0; s < steps; ++s) {
-        orc_gravitation_f64(n, pos4, eps2, f);
+        grav_f64_threaded(n, pos4, eps2, f, nthreads);
         orc_integrate_f64(n, pos4, vel4, f, dt, damping);
     }
     free(f);
     return 0;
+}
+
+int orc_run_f64(int n, int steps, double* pos4, double* vel4, double eps2, double damping,
+                double dt)
+{
+    return orc_run_f64_mt(n, steps, pos4, vel4, eps2, damping, dt, 1);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -40,6 +40,10 @@ int orc_run_f32(int n, int steps, float* pos4, float* vel4, float eps2, float da
                 float dt, int nthreads);
 int orc_run_f64(int n, int steps, double* pos4, double* vel4, double eps2,
                 double damping, double dt);
+/* the same on nthreads worker threads (rows split by the reference partition
+ * rule; bit-identical to orc_run_f64 for any thread count) */
+int orc_run_f64_mt(int n, int steps, double* pos4, double* vel4, double eps2,
+                   double damping, double dt, int nthreads);
 
 /* ---- initial conditions (glibc rand(), as the reference) --------------- */
 void orc_srand(unsigned seed);

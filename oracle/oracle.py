@@ -40,6 +40,7 @@ def lib():
         o.orc_integrate_f64.argtypes = [C.c_int, _vp, _vp, _vp, _d, _d]
         o.orc_run_f32.argtypes = [C.c_int, C.c_int, _vp, _vp, _f, _f, _f, C.c_int]
         o.orc_run_f64.argtypes = [C.c_int, C.c_int, _vp, _vp, _d, _d, _d]
+        o.orc_run_f64_mt.argtypes = [C.c_int, C.c_int, _vp, _vp, _d, _d, _d, C.c_int]
         o.orc_srand.argtypes = [C.c_uint]
         o.orc_randomize_f32.argtypes = [C.c_int, _vp, _vp, _vp, _f, _f, C.c_int, C.c_int]
         o.orc_randomize_f64.argtypes = [C.c_int, _vp, _vp, _vp, _f, _f, C.c_int, C.c_int]
@@ -104,10 +105,10 @@ def run_f32(pos4, vel4, steps, eps2, damping, dt, nthreads=1):
     return p, v
 
 
-def run_f64(pos4, vel4, steps, eps2, damping, dt):
+def run_f64(pos4, vel4, steps, eps2, damping, dt, nthreads=1):
     p = np.ascontiguousarray(pos4, np.float64).copy()
     v = np.ascontiguousarray(vel4, np.float64).copy()
-    lib().orc_run_f64(len(p), steps, _ptr(p), _ptr(v), _d(eps2), _d(damping), _d(dt))
+    lib().orc_run_f64_mt(len(p), steps, _ptr(p), _ptr(v), _d(eps2), _d(damping), _d(dt), nthreads)
     return p, v
 
 

@@ -5,13 +5,17 @@ output of `-n 1024 -l 100 -O file` with FP32 arithmetic; nbody_main must
 reproduce it byte for byte with the reference arithmetic, and stay within the
 reference's own GPU-vs-CPU envelope with the fast precisions.
 """
+import ctypes as C
 import hashlib
+import json
 import subprocess
 import sys
 from pathlib import Path
 
 import numpy as np
 import pytest
+
+from oracle import oracle as O
 
 ROOT = Path(__file__).resolve().parent.parent
 BIN = ROOT / "utc_amd" / "lib"
@@ -71,6 +75,64 @@ def test_nbody_main_double_frames(tmp_path):
     # the reference labels frame k with k * timestep, not with the step count
     # (benchapps/nbody/utc/nbody_main.cc:236-238)
     assert lines[0] == "0.000000" and lines[n + 1] == "0.016000" and lines[4 * (n + 1)] == "0.064000"
+
+
+PRESETS = json.loads((GOLDEN / "preset_frames_fp64.json").read_text())
+
+
+@pytest.mark.parametrize("n", sorted({r["n"] for r in PRESETS["frames"]}))
+def test_nbody_main_presets_reproduce_reference_driver(tmp_path, n):
+    """The UTC driver (FP64, the reference's arithmetic on the GPU) for every
+    preset -i 0..6 at N: byte for byte the frame file of the reference's own
+    sequential driver (tests/golden/preset_frames_fp64.json)."""
+    for rec in [r for r in PRESETS["frames"] if r["n"] == n]:
+        out = tmp_path / f"p{rec['preset']}.txt"
+        run([str(BIN / "nbody_main"), "-n", str(n), "-i", str(rec["preset"]), "-l", str(rec["steps"]),
+             "-O", str(out)], tmp_path)
+        assert hashlib.sha256(out.read_bytes()).hexdigest() == rec["sha256"], rec["preset"]
+
+
+def _mtask_oracle_frames(ntasks, n, steps, dtype):
+    """The oracle's runs of the multi-task driver's systems: task i starts from
+    the i-th consecutive SHELL set of one rand() stream (as NbodyInit runs task
+    after task, benchapps/nbody/utc/nbody_mtask.cc:185-199)."""
+    prm = O.demo_params(0, n)
+    O.lib().orc_srand(1)
+    f = O.lib().orc_randomize_f32 if dtype == np.float32 else O.lib().orc_randomize_f64
+    out = []
+    for _ in range(ntasks):
+        pos = np.zeros((n, 4), dtype)
+        vel = np.zeros((n, 4), dtype)
+        f(1, O._ptr(pos), O._ptr(vel), None, C.c_float(prm["clusterScale"]), C.c_float(prm["velocityScale"]),
+          n, 1)
+        eps2 = dtype(prm["softening"]) * dtype(prm["softening"])
+        if dtype == np.float32:
+            p, _ = O.run_f32(pos, vel, steps, eps2, dtype(prm["damping"]), dtype(prm["timestep"]), nthreads=16)
+        else:
+            p, _ = O.run_f64(pos, vel, steps, eps2, float(prm["damping"]), float(prm["timestep"]), nthreads=16)
+        out.append((pos, p))
+    return out, prm
+
+
+def test_nbody_mtask_tasks_match_oracle(tmp_path):
+    """-T 4 -n 4096 -l 10 in FP32: each task's frame file is byte for byte the
+    oracle's run of that task's initial conditions (the reference arithmetic);
+    with -P 2 (mixed) every task stays within 2e-5 of it (two steps of the
+    frame file's %.5f)."""
+    T, n, steps = 4, 4096, 10
+    ref, prm = _mtask_oracle_frames(T, n, steps, np.float32)
+    run([str(BIN / "nbody_mtask"), "-f", "-T", str(T), "-n", str(n), "-l", str(steps), "-O", "ref"], tmp_path)
+    run([str(BIN / "nbody_mtask"), "-f", "-T", str(T), "-n", str(n), "-l", str(steps), "-P", "2", "-O", "mix"],
+        tmp_path)
+    for i, (p0, p1) in enumerate(ref):
+        want = tmp_path / f"oracle.{i}"
+        O.write_frames(want, np.stack([p0, p1]), prm["timestep"])
+        assert (tmp_path / f"ref.{i}").read_bytes() == want.read_bytes(), i
+        got = np.array([[float(x) for x in l.split()] for l in
+                        (tmp_path / f"mix.{i}").read_text().splitlines()[n + 2:2 * n + 2]])
+        assert np.abs(got - p1).max() <= 2e-5, i
+    # the tasks' systems differ (successive rand() streams)
+    assert not np.array_equal(ref[0][0], ref[1][0])
 
 
 def test_nbody_mtask_runs_concurrent_tasks(tmp_path):

@@ -1,0 +1,225 @@
+"""GPU tests of the target-slice path: one GPU's share of a system sharded over
+targets, as the reference's multi-GPU nbody runs it
+(benchapps/nbody/utc-snode-mgpu/mgpu/body_task_mgpu.cu:41-49 partition,
+bodysystem_kernel.cu:88-140 slice kernel).
+
+* the kernel-level slice step (utcnb_integrate_slice_f32/f64, slice-local
+  pos_new and vel) is bit-identical to the oracle's rows of the reference's
+  force loop and integrator, on uneven slices of N = 262147;
+* slices covering [0, n) and a commit equal one full step with the same
+  one-pass tile, bit for bit, in every precision;
+* four handles on one GPU, each stepping its own slice and receiving the
+  others' slices into its next position buffer (the all-gather of a sharded
+  run), end bitwise where one handle's full steps end;
+* the slice field against the CPU FP64 oracle's rows.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import utc_amd as U
+from nbcases import plummer, relnorm
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_partition_rule():
+    assert U.partition(262147, 8)[0] == (0, 32769)
+    assert U.partition(262147, 8)[3] == (3 * 32769, 32768)
+    assert U.partition(262147, 8)[7] == (262147 - 32768, 32768)
+    parts = U.partition(1001, 7)
+    assert sum(c for _, c in parts) == 1001
+    assert all(parts[k][0] + parts[k][1] == parts[k + 1][0] for k in range(6))
+
+
+@pytest.mark.parametrize("which", [0, 7])
+def test_kernel_level_slice_bit_exact_f32(which):
+    """N = 262147 over 8 shares (the first three take 32769 bodies, the rest
+    32768): share 0 and share 7 in the reference's FP32 arithmetic against the
+    oracle's rows, bit for bit."""
+    import torch
+    n = 262147
+    start, cnt = U.partition(n, 8)[which]
+    c = plummer(n, eps=0.002, dt=2.5e-4)
+    pos = c["pos"].astype(np.float32)
+    vel = c["vel"].astype(np.float32)
+    eps2 = np.float32(0.002) * np.float32(0.002)
+    dt, damping = np.float32(2.5e-4), np.float32(0.995)
+    say(f"case: kernel-level slice f32 n={n} share={which} start={start} count={cnt}")
+    d_old = torch.from_numpy(pos).cuda()
+    d_new = torch.zeros((cnt, 4), dtype=torch.float32, device="cuda")
+    d_vel = torch.from_numpy(vel[start:start + cnt].copy()).cuda()
+    torch.cuda.synchronize()
+    U.check(U.lib().utcnb_integrate_slice_f32(d_old.data_ptr(), d_new.data_ptr(), d_vel.data_ptr(), n, start,
+                                              cnt, dt, eps2, damping, 0, None), "utcnb_integrate_slice_f32")
+    torch.cuda.synchronize()
+    f = O.gravitation_rows_f32(pos, eps2, start, start + cnt, nthreads=16)[start:start + cnt].copy()
+    op, ov = pos[start:start + cnt].copy(), vel[start:start + cnt].copy()
+    O.lib().orc_integrate_f32(cnt, O._ptr(op), O._ptr(ov), O._ptr(f), C.c_float(dt), C.c_float(damping))
+    assert np.array_equal(d_new.cpu().numpy(), op)
+    assert np.array_equal(d_vel.cpu().numpy(), ov)
+
+
+def test_kernel_level_slice_bit_exact_f64():
+    import torch
+    n = 5003
+    c = plummer(n, eps=0.01, dt=1e-3)
+    pos, vel = c["pos"], c["vel"]
+    eps2, dt, damping = 1e-4, 1e-3, 0.995
+    f = O.gravitation_f64(pos, eps2)
+    d_old = torch.from_numpy(pos).cuda()
+    for start, cnt in U.partition(n, 3):
+        say(f"case: kernel-level slice f64 n={n} start={start} count={cnt}")
+        d_new = torch.zeros((cnt, 4), dtype=torch.float64, device="cuda")
+        d_vel = torch.from_numpy(vel[start:start + cnt].copy()).cuda()
+        torch.cuda.synchronize()
+        U.check(U.lib().utcnb_integrate_slice_f64(d_old.data_ptr(), d_new.data_ptr(), d_vel.data_ptr(), n,
+                                                  start, cnt, dt, eps2, damping, 128, None),
+                "utcnb_integrate_slice_f64")
+        torch.cuda.synchronize()
+        op, ov = pos[start:start + cnt].copy(), vel[start:start + cnt].copy()
+        fs = f[start:start + cnt].copy()
+        O.lib().orc_integrate_f64(cnt, O._ptr(op), O._ptr(ov), O._ptr(fs), C.c_double(dt), C.c_double(damping))
+        assert np.array_equal(d_new.cpu().numpy(), op)
+        assert np.array_equal(d_vel.cpu().numpy(), ov)
+
+
+def test_kernel_level_slice_rejects_bad_ranges():
+    L = U.lib()
+    for start, cnt in ((-1, 4), (0, 0), (8, 3), (5, 6)):
+        assert L.utcnb_integrate_slice_f32(1, 2, 3, 10, start, cnt, 0.1, 0.1, 1.0, 0, None) == U.EINVAL
+    assert b"start" in L.utcnb_last_error()
+
+
+CASES = [
+    ("mixed", U.PREC_MIXED, "k1s16", U.F64),
+    ("fp32", U.PREC_FP32, "k2s8", U.F64),
+    ("fp64", U.PREC_FP64, None, U.F64),
+    ("ref_f32", U.PREC_REFERENCE, None, U.F32),
+    ("ref_f64", U.PREC_REFERENCE, None, U.F64),
+]
+
+
+@pytest.mark.parametrize("label,prec,tile,sdt", CASES)
+def test_slices_then_commit_equal_full_steps(label, prec, tile, sdt):
+    n, steps, shares = 5003, 3, 5
+    c = plummer(n, eps=0.01, dt=1e-3)
+    kick = U.KICK_REFERENCE if prec == U.PREC_REFERENCE else U.KICK_PHYSICAL
+    tid = 0 if tile is None else U.tiles().index(tile) + 1
+    pos = c["pos"].astype(np.float32) if sdt == U.F32 else c["pos"]
+    vel = c["vel"].astype(np.float32) if sdt == U.F32 else c["vel"]
+    dtype = np.float32 if sdt == U.F32 else np.float64
+    say(f"case: slices+commit vs full steps {label} n={n}")
+    out = []
+    for sliced in (False, True):
+        with U.System(n, prec, 1e-4, 1e-3, 0.999, kick=kick, state_dtype=sdt, tile=tid) as s:
+            s.set_state(pos, vel)
+            if tile is not None:
+                assert s.slice_tile(100)[1] == tile
+            for _ in range(steps):
+                if sliced:
+                    for start, cnt in U.partition(n, shares):
+                        s.step_slice(start, cnt)
+                    s.commit_step()
+                else:
+                    s.step(1)
+            out.append(s.get_state(dtype))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def _hip():
+    h = C.CDLL("libamdhip64.so")
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipMemcpy.restype = C.c_int
+    h.hipDeviceSynchronize.restype = C.c_int
+    return h
+
+
+def test_sharded_handles_with_all_gather_equal_one_system():
+    """Four MIXED handles on one GPU stand for four GPUs: each steps its own
+    share, then receives every other share's new FP32 positions into its next
+    position buffer (what an all-gather over the GPUs would do) and commits.
+    After 4 steps the shares' FP64 states, put together, are bitwise the state
+    of one handle running full steps with the same one-pass tile."""
+    n, G, steps = 20001, 4, 4
+    c = plummer(n, eps=0.005, dt=5e-4)
+    tid = U.tiles().index("k1s16") + 1
+    parts = U.partition(n, G)
+    hip = _hip()
+    say(f"case: {G} sharded handles n={n}")
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], tile=tid) as one:
+        one.set_state(c["pos"], c["vel"])
+        one.step(steps)
+        ref_p, ref_v = one.get_state()
+    hs = [U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], tile=tid) for _ in range(G)]
+    try:
+        for h in hs:
+            h.set_state(c["pos"], c["vel"])
+        for _ in range(steps):
+            for h, (start, cnt) in zip(hs, parts):
+                h.step_slice(start, cnt)
+            for h in hs:
+                h.sync()
+            for g, h in enumerate(hs):  # the all-gather of the new snapshots
+                dst, dt = h.position_buffer(1)
+                assert dt == U.F32
+                for r, (start, cnt) in enumerate(parts):
+                    if r == g:
+                        continue
+                    src, _ = hs[r].position_buffer(1)
+                    assert hip.hipMemcpy(dst + 16 * start, src + 16 * start, 16 * cnt, 3) == 0
+            assert hip.hipDeviceSynchronize() == 0
+            for h in hs:
+                h.commit_step()
+        got_p = np.zeros_like(ref_p)
+        got_v = np.zeros_like(ref_v)
+        for h, (start, cnt) in zip(hs, parts):
+            p, v = h.get_state()
+            got_p[start:start + cnt] = p[start:start + cnt]
+            got_v[start:start + cnt] = v[start:start + cnt]
+    finally:
+        for h in hs:
+            h.close()
+    assert np.array_equal(got_p, ref_p)
+    assert np.array_equal(got_v, ref_v)
+
+
+@pytest.mark.parametrize("which", [0, 7])
+def test_slice_field_against_cpu_fp64_oracle(which):
+    """The field of share 0 / share 7 of N = 262147 (FP64, mixed and plain
+    FP32 on the slice's one-pass tile) against the CPU FP64 oracle's rows
+    (512 sampled); the FP64 and one-pass slice fields equal the rows of the
+    full evaluation bit for bit."""
+    n = 262147
+    start, cnt = U.partition(n, 8)[which]
+    c = plummer(n, eps=0.002, dt=2.5e-4)
+    eps2 = c["eps"] ** 2
+    rows = np.sort(np.random.default_rng(which).choice(cnt, 512, replace=False))
+    ref = O.accel_rows_f64(c["pos"], eps2, start + rows, nthreads=16)
+    say(f"case: slice field vs CPU FP64 n={n} share={which}")
+    errs = {}
+    for label, prec in (("fp64", U.PREC_FP64), ("mixed", U.PREC_MIXED), ("fp32", U.PREC_FP32)):
+        with U.System(n, prec, eps2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+            s.set_state(c["pos"], c["vel"])
+            a = s.accel_slice(start, cnt)
+            errs[label] = relnorm(a[rows], ref)
+            if prec == U.PREC_FP64:
+                assert np.array_equal(a, s.accel()[start:start + cnt])
+        if prec != U.PREC_FP64:
+            tid = U.tiles().index("k1s16") + 1
+            with U.System(n, prec, eps2, c["dt"], flags=U.FLAG_NO_NEWTON, tile=tid) as s:
+                s.set_state(c["pos"], c["vel"])
+                assert np.array_equal(s.accel_slice(start, cnt), s.accel()[start:start + cnt])
+    say(f"  rel err: {errs}")
+    assert errs["fp64"] <= 1e-14
+    assert errs["mixed"] <= 2e-6
+    assert errs["fp32"] <= 4e-6

@@ -5,6 +5,7 @@ No GPU needed.  The oracle is the checker for every GPU test, so each of its
 functions is pinned here by something other than a retyped formula.
 """
 import hashlib
+import json
 import math
 from pathlib import Path
 
@@ -85,6 +86,27 @@ def test_oracle_matches_reference_build(config, dtype):
                       C.c_double(0.995), C.c_double(0.016))
         gp, gv = O.run_f64(pos, vel, 7, 0.1 * 0.1, 0.995, 0.016)
     assert np.array_equal(rp, gp) and np.array_equal(rv, gv)
+
+
+PRESETS = json.loads((Path(__file__).resolve().parent / "golden" / "preset_frames_fp64.json").read_text())
+
+
+def test_preset_table_reproduces_reference_driver_frames(tmp_path):
+    """Every preset (-i 0..6) at N = 1024 ... 33024 (each N branch of the
+    cluster / velocity scales, and one N above them): the oracle's preset
+    table, SHELL generator, FP64 steps and frame writer reproduce, byte for
+    byte, the frame files of the reference's own sequential driver
+    (tests/golden/preset_frames_fp64.json, made by make_preset_fixtures.py)."""
+    for rec in PRESETS["frames"]:
+        n, preset, steps = rec["n"], rec["preset"], rec["steps"]
+        prm = O.demo_params(preset, n)
+        pos, vel = O.randomize(1, n, prm["clusterScale"], prm["velocityScale"], dtype=np.float64)
+        eps2 = float(prm["softening"]) * float(prm["softening"])
+        p2, _ = O.run_f64(pos, vel, steps, eps2, float(prm["damping"]), float(prm["timestep"]), nthreads=8)
+        path = tmp_path / f"p{preset}_{n}.txt"
+        O.write_frames(path, np.stack([pos, p2]), prm["timestep"])
+        data = path.read_bytes()
+        assert hashlib.sha256(data).hexdigest() == rec["sha256"], (preset, n)
 
 
 def test_threaded_rows_equal_serial():

@@ -35,6 +35,8 @@ EXPORTS = [
     "utcnb_batch_run", "utcnb_batch_get_tile", "utcnb_autotune", "utcnb_tile_count", "utcnb_tile_name",
     "utcnb_rng_seed", "utcnb_rng_next", "utcnb_splitmix64", "utcnb_generate",
     "utcnb_last_error", "utcnb_version", "utcnb_device_info", "utcnb_accel_noalloc",
+    "utcnb_integrate_slice_f32", "utcnb_integrate_slice_f64", "utcnb_step_slice", "utcnb_commit_step",
+    "utcnb_position_buffer", "utcnb_accel_slice", "utcnb_slice_tile",
 ]
 
 
@@ -95,6 +97,15 @@ def _proto(lib):
         "utcnb_version": (C.c_char_p, []),
         "utcnb_device_info": (C.c_int, [C.c_char_p, C.c_int, P(C.c_int), P(C.c_int)]),
         "utcnb_accel_noalloc": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+        "utcnb_integrate_slice_f32": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                                C.c_float, C.c_int, _vp]),
+        "utcnb_integrate_slice_f64": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                C.c_double, C.c_int, _vp]),
+        "utcnb_step_slice": (C.c_int, [_vp, C.c_int, C.c_int]),
+        "utcnb_commit_step": (C.c_int, [_vp]),
+        "utcnb_position_buffer": (C.c_int, [_vp, C.c_int, P(_vp), P(C.c_int)]),
+        "utcnb_accel_slice": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+        "utcnb_slice_tile": (C.c_int, [_vp, C.c_int, P(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):
@@ -198,6 +209,29 @@ class System:
         check(lib().utcnb_diagnostics(self.h, C.byref(k), C.byref(w), ptr(p)), "utcnb_diagnostics")
         return k.value, w.value, p
 
+    def step_slice(self, start: int, count: int):
+        check(lib().utcnb_step_slice(self.h, start, count), "utcnb_step_slice")
+
+    def commit_step(self):
+        check(lib().utcnb_commit_step(self.h), "utcnb_commit_step")
+
+    def position_buffer(self, which: int = 0):
+        """(device address, dtype) of the current (0) or next (1) position buffer."""
+        p, dt = C.c_void_p(), C.c_int()
+        check(lib().utcnb_position_buffer(self.h, which, C.byref(p), C.byref(dt)), "utcnb_position_buffer")
+        return p.value, dt.value
+
+    def accel_slice(self, start: int, count: int):
+        a = np.zeros((count, 3), np.float64)
+        check(lib().utcnb_accel_slice(self.h, start, count, ptr(a)), "utcnb_accel_slice")
+        return a
+
+    def slice_tile(self, count: int):
+        """(tile id, name) that the slice calls run for `count` targets."""
+        t = C.c_int()
+        check(lib().utcnb_slice_tile(self.h, count, C.byref(t)), "utcnb_slice_tile")
+        return t.value, (tiles()[t.value - 1] if t.value > 0 else "")
+
     def last_kernel_ms(self):
         ms, cnt = C.c_double(), C.c_int()
         check(lib().utcnb_last_kernel_ms(self.h, C.byref(ms), C.byref(cnt)), "utcnb_last_kernel_ms")
@@ -220,6 +254,19 @@ class System:
         check(lib().utcnb_adjoint(self.h, nsteps, ckpt_every, ptr(t), C.byref(loss), ptr(dx), ptr(dv), ptr(dm)),
               "utcnb_adjoint")
         return loss.value, dx, dv, dm
+
+
+def partition(n: int, parts: int):
+    """[(start, count)] of the reference's body partition over `parts` shares:
+    the first n % parts shares take one body more
+    (benchapps/nbody/utc-snode-mgpu/mgpu/body_task_mgpu.cu:41-49)."""
+    per, extra = divmod(n, parts)
+    out = []
+    for r in range(parts):
+        cnt = per + 1 if r < extra else per
+        start = r * (per + 1) if r < extra else r * per + extra
+        out.append((start, cnt))
+    return out
 
 
 def device_info():

@@ -5,7 +5,9 @@
 // stream each); all run() calls are issued, then all wait() calls; the real
 // wall time of the whole set is reported beside the per-task averages, and
 // print_time appends {avg total, avg kernel, avg copy-in, avg copy-out, real
-// total} (ms).  Options as nbody_main plus -T (number of tasks, <= 48).
+// total} (ms).  Options as nbody_main plus -T (number of tasks, <= 48).  The
+// reference parses -O here but writes nothing; this driver writes task i's
+// frames to "<-O argument>.<i>" in the frame-file format of nbody_main.
 #include <getopt.h>
 
 #include <cstdio>
@@ -15,6 +17,7 @@
 #include <iostream>
 #include <array>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "Utc.h"
@@ -28,7 +31,7 @@ using namespace iUtc;
 
 template <typename FTYPE>
 static int run(bool printTime, int paraSelect, int iteration, int outInterval, int numBodies,
-               int blocksize, MemType memtype, int ntasks, int precision)
+               int blocksize, MemType memtype, int ntasks, int precision, const char* outPrefix)
 {
     if (numBodies % blocksize != 0)
         numBodies = (numBodies / blocksize + 1) * blocksize;
@@ -64,6 +67,23 @@ static int run(bool printTime, int paraSelect, int iteration, int outInterval, i
     for (int i = 0; i < ntasks; ++i)
         tasks[i]->wait();
     double totaltime = timer.stop();
+    if (outPrefix) {
+        Task<Output<FTYPE>> outtask(ProcList(0));
+        for (int i = 0; i < ntasks; ++i) {
+            const std::string name = std::string(outPrefix) + "." + std::to_string(i);
+            FILE* fp = std::fopen(name.c_str(), "w");
+            if (!fp) {
+                std::cerr << "cannot open " << name << std::endl;
+                return 1;
+            }
+            for (int f = 0; f < nframes; ++f) {
+                FTYPE timestamp = f * timestep;
+                outtask.run(&fp, &out[i][(size_t)f * numBodies * 4], (float)timestamp, numBodies);
+                outtask.wait();
+            }
+            std::fclose(fp);
+        }
+    }
     double runtime[5] = {0, 0, 0, 0, 0};
     for (int i = 0; i < ntasks; ++i)
         for (int j = 0; j < 4; ++j)
@@ -98,6 +118,7 @@ int main(int argc, char** argv)
     int iteration = 1, outInterval = 0, numBodies = 1024, blocksize = 256, mtype = 0, ntasks = 1;
     int paraSelect = 0;
     int precision = UTCNB_PREC_REFERENCE;
+    const char* outPrefix = nullptr;
     UtcContext::getContext(argc, argv);
     int opt;
     while ((opt = getopt(argc, argv, "vt:p:m:i:l:o:O:n:b:T:P:f")) != EOF) {
@@ -110,6 +131,7 @@ int main(int argc, char** argv)
         case 'n': numBodies = std::atoi(optarg); break;
         case 'b': blocksize = std::atoi(optarg); break;
         case 'T': ntasks = std::atoi(optarg); break;
+        case 'O': outPrefix = optarg; break;
         case 'P': precision = std::atoi(optarg); break;
         case 'f': useFloat = true; break;
         default: break;
@@ -124,6 +146,8 @@ int main(int argc, char** argv)
         outInterval = iteration;
     MemType memtype = mtype == 1 ? MemType::pinned : (mtype == 2 ? MemType::unified : MemType::pageable);
     if (useFloat)
-        return run<float>(printTime, paraSelect, iteration, outInterval, numBodies, blocksize, memtype, ntasks, precision);
-    return run<double>(printTime, paraSelect, iteration, outInterval, numBodies, blocksize, memtype, ntasks, precision);
+        return run<float>(printTime, paraSelect, iteration, outInterval, numBodies, blocksize, memtype, ntasks,
+                          precision, outPrefix);
+    return run<double>(printTime, paraSelect, iteration, outInterval, numBodies, blocksize, memtype, ntasks,
+                       precision, outPrefix);
 }

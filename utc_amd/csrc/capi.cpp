@@ -303,6 +303,45 @@ int utcnb_integrate_f64(const double* pos_old, double* pos_new, double* vel, int
     return UTCNB_OK;
 }
 
+static int check_slice_args(const char* who, int numBodies, int start, int count)
+{
+    if (numBodies <= 0 || start < 0 || count < 1 || (int64_t)start + count > numBodies)
+        return fail(UTCNB_EINVAL, std::string(who) + ": need 0 <= start, count >= 1, start + count <= numBodies");
+    return UTCNB_OK;
+}
+
+int utcnb_integrate_slice_f32(const float* pos_old, float* pos_new, float* vel, int numBodies, int start,
+                              int count, float deltaTime, float softeningSquared, float damping, int blocksize,
+                              void* stream)
+{
+    if (!pos_old || !pos_new || !vel)
+        return fail(UTCNB_EINVAL, "utcnb_integrate_slice_f32: null pointer");
+    if (int rc = check_slice_args("utcnb_integrate_slice_f32", numBodies, start, count))
+        return rc;
+    if (blocksize != 0 && (blocksize < 64 || blocksize > 1024 || blocksize % 64 != 0))
+        return fail(UTCNB_EINVAL, "utcnb_integrate_slice_f32: blocksize must be 64..1024, multiple of 64");
+    UTCNB_HIP(launch_ref_step_slice_f32((const float4*)pos_old, (float4*)pos_new, (float4*)vel, numBodies, start,
+                                        count, deltaTime, softeningSquared, damping, blocksize,
+                                        (hipStream_t)stream));
+    return UTCNB_OK;
+}
+
+int utcnb_integrate_slice_f64(const double* pos_old, double* pos_new, double* vel, int numBodies, int start,
+                              int count, double deltaTime, double softeningSquared, double damping,
+                              int blocksize, void* stream)
+{
+    if (!pos_old || !pos_new || !vel)
+        return fail(UTCNB_EINVAL, "utcnb_integrate_slice_f64: null pointer");
+    if (int rc = check_slice_args("utcnb_integrate_slice_f64", numBodies, start, count))
+        return rc;
+    if (blocksize != 0 && (blocksize < 64 || blocksize > 1024 || blocksize % 64 != 0))
+        return fail(UTCNB_EINVAL, "utcnb_integrate_slice_f64: blocksize must be 64..1024, multiple of 64");
+    UTCNB_HIP(launch_ref_step_slice_f64((const double4*)pos_old, (double4*)pos_new, (double4*)vel, numBodies,
+                                        start, count, deltaTime, softeningSquared, damping, blocksize,
+                                        (hipStream_t)stream));
+    return UTCNB_OK;
+}
+
 int utcnb_tile_count(void) { return tile_count(); }
 
 const char* utcnb_tile_name(int tile)
@@ -597,6 +636,138 @@ int utcnb_last_kernel_ms(utcnb_system* s, double* ms, int* launches)
     *ms = f;
     if (launches)
         *launches = s->last_launches;
+    return UTCNB_OK;
+}
+
+// One-pass tile of a slice of `count` targets (0 for REFERENCE / FP64).
+static int slice_tile_of(const utcnb_system* s, int count)
+{
+    if (s->cfg.precision != UTCNB_PREC_FP32 && s->cfg.precision != UTCNB_PREC_MIXED)
+        return -1;
+    return tile_get(s->args.tile).sym ? slice_tile_default(count, s->cu_count) : s->args.tile;
+}
+
+int utcnb_slice_tile(const utcnb_system* s, int count, int* tile)
+{
+    if (!s || !tile || count < 1)
+        return fail(UTCNB_EINVAL, "utcnb_slice_tile: null argument or count < 1");
+    *tile = slice_tile_of(s, count) + 1;
+    return UTCNB_OK;
+}
+
+int utcnb_step_slice(utcnb_system* s, int start, int count)
+{
+    if (!s)
+        return fail(UTCNB_EINVAL, "utcnb_step_slice: null handle");
+    if (int rc = check_slice_args("utcnb_step_slice", s->n, start, count))
+        return rc;
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_step_slice: no state set");
+    const int c = s->cur;
+    const int t = slice_tile_of(s, count);
+    hipError_t e = hipSuccess;
+    UTCNB_HIP(hipEventRecord(s->ev0, s->stream));
+    switch (s->cfg.precision) {
+    case UTCNB_PREC_REFERENCE:
+        if (s->cfg.state_dtype == UTCNB_F32)
+            e = launch_ref_step_slice_f32(s->p32[c], s->p32[1 - c] + start, s->v32 + start, s->n, start, count,
+                                          (float)s->cfg.dt, (float)s->cfg.eps2, (float)s->cfg.damping, 256,
+                                          s->stream);
+        else
+            e = launch_ref_step_slice_f64(s->p64[c], s->p64[1 - c] + start, s->v64 + start, s->n, start, count,
+                                          s->cfg.dt, s->cfg.eps2, s->cfg.damping, 256, s->stream);
+        break;
+    case UTCNB_PREC_FP32:
+        e = launch_fp32_step_slice(s->args, t, start, count, s->p32[c], s->p32[1 - c], s->v32, s->stream);
+        break;
+    case UTCNB_PREC_MIXED:
+        e = launch_mixed_step_slice(s->args, t, start, count, s->snap[c], s->snap[1 - c], s->p64[0], s->v64,
+                                    s->stream);
+        break;
+    case UTCNB_PREC_FP64:
+        e = launch_fp64_step_slice(s->args, start, count, s->p64[c], s->p64[1 - c], s->v64, s->stream);
+        break;
+    }
+    if (e != hipSuccess)
+        return fail(UTCNB_EHIP, std::string("utcnb_step_slice: ") + hipGetErrorString(e));
+    UTCNB_HIP(hipEventRecord(s->ev1, s->stream));
+    s->last_launches = 1;
+    s->timed = true;
+    return UTCNB_OK;
+}
+
+int utcnb_commit_step(utcnb_system* s)
+{
+    if (!s)
+        return fail(UTCNB_EINVAL, "utcnb_commit_step: null handle");
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_commit_step: no state set");
+    s->cur = 1 - s->cur;
+    return UTCNB_OK;
+}
+
+int utcnb_position_buffer(utcnb_system* s, int which, void** pos4, int* dtype)
+{
+    if (!s || !pos4 || (which != 0 && which != 1))
+        return fail(UTCNB_EINVAL, "utcnb_position_buffer: null argument or which not 0/1");
+    const int b = which == 0 ? s->cur : 1 - s->cur;
+    int dt = UTCNB_F32;
+    void* p = nullptr;
+    switch (s->cfg.precision) {
+    case UTCNB_PREC_REFERENCE:
+        if (s->cfg.state_dtype == UTCNB_F32) {
+            p = s->p32[b];
+        } else {
+            p = s->p64[b];
+            dt = UTCNB_F64;
+        }
+        break;
+    case UTCNB_PREC_FP32: p = s->p32[b]; break;
+    case UTCNB_PREC_MIXED: p = s->snap[b]; break;
+    case UTCNB_PREC_FP64:
+        p = s->p64[b];
+        dt = UTCNB_F64;
+        break;
+    }
+    *pos4 = p;
+    if (dtype)
+        *dtype = dt;
+    return UTCNB_OK;
+}
+
+int utcnb_accel_slice(utcnb_system* s, int start, int count, double* acc3)
+{
+    if (!s || !acc3)
+        return fail(UTCNB_EINVAL, "utcnb_accel_slice: null argument");
+    if (int rc = check_slice_args("utcnb_accel_slice", s->n, start, count))
+        return rc;
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_accel_slice: no state set");
+    const int t = slice_tile_of(s, count);
+    hipError_t e = hipSuccess;
+    switch (s->cfg.precision) {
+    case UTCNB_PREC_REFERENCE:
+        if (s->cfg.state_dtype == UTCNB_F32)
+            e = launch_ref_accel_slice_f32(s->p32[s->cur], s->n, start, count, (float)s->cfg.eps2, s->scratch,
+                                           s->stream);
+        else
+            e = launch_ref_accel_slice_f64(s->p64[s->cur], s->n, start, count, s->cfg.eps2, s->scratch, s->stream);
+        break;
+    case UTCNB_PREC_FP32:
+        e = launch_fp32_accel_slice(s->args, t, start, count, s->p32[s->cur], false, s->scratch, s->stream);
+        break;
+    case UTCNB_PREC_MIXED:
+        e = launch_fp32_accel_slice(s->args, t, start, count, s->snap[s->cur], true, s->scratch, s->stream);
+        break;
+    case UTCNB_PREC_FP64:
+        e = launch_fp64_accel_slice(s->args, start, count, s->p64[s->cur], s->scratch, s->stream);
+        break;
+    }
+    if (e != hipSuccess)
+        return fail(UTCNB_EHIP, std::string("utcnb_accel_slice: ") + hipGetErrorString(e));
+    UTCNB_HIP(hipMemcpyAsync(acc3, s->scratch, sizeof(double) * 3 * (size_t)count, hipMemcpyDeviceToHost,
+                             s->stream));
+    UTCNB_HIP(hipStreamSynchronize(s->stream));
     return UTCNB_OK;
 }
 

@@ -99,6 +99,51 @@ hipError_t launch_mixed_step(const StepArgs& a, const float4* snap_old, float4* 
     return launch_force<true, false>(a.tile, snap_old, a.n, a.n_pad, a.eps2f, epi, st);
 }
 
+// One-pass tile for a slice of nt targets against all sources: the symmetric
+// tiles evaluate unordered pairs of the whole system and do not split by
+// target, so a slice takes k2s8 when its tiles give every CU 64 waves and
+// k1s16 (twice the waves per target) otherwise.
+int slice_tile_default(int64_t nt, int cu_count)
+{
+    const int64_t cus = cu_count > 0 ? cu_count : 256;
+    return (nt + 127) / 128 * 8 >= cus * 64 ? 8 : 9;
+}
+
+hipError_t launch_fp32_step_slice(const StepArgs& a, int tile, int i0, int nt, const float4* pos_old,
+                                  float4* pos_new, float4* vel, hipStream_t st)
+{
+    if (tile_get(tile).sym)
+        return hipErrorInvalidValue;
+    EpiFP32 epi{pos_old, pos_new, vel, (float)a.dt, (float)a.damping, a.kick};
+    if (a.newton)
+        return launch_force_range<false, true>(tile, pos_old, i0, nt, a.n_pad, a.eps2f, epi, st);
+    return launch_force_range<false, false>(tile, pos_old, i0, nt, a.n_pad, a.eps2f, epi, st);
+}
+
+hipError_t launch_mixed_step_slice(const StepArgs& a, int tile, int i0, int nt, const float4* snap_old,
+                                   float4* snap_new, double4* pos, double4* vel, hipStream_t st)
+{
+    if (tile_get(tile).sym)
+        return hipErrorInvalidValue;
+    EpiMixed epi{pos, vel, snap_new, a.dt, a.damping, a.kick};
+    if (a.newton)
+        return launch_force_range<true, true>(tile, snap_old, i0, nt, a.n_pad, a.eps2f, epi, st);
+    return launch_force_range<true, false>(tile, snap_old, i0, nt, a.n_pad, a.eps2f, epi, st);
+}
+
+hipError_t launch_fp32_accel_slice(const StepArgs& a, int tile, int i0, int nt, const float4* src,
+                                   bool compensated, double* acc3, hipStream_t st)
+{
+    if (tile_get(tile).sym)
+        return hipErrorInvalidValue;
+    EpiAccel epi{acc3, i0};
+    if (compensated)
+        return a.newton ? launch_force_range<true, true>(tile, src, i0, nt, a.n_pad, a.eps2f, epi, st)
+                        : launch_force_range<true, false>(tile, src, i0, nt, a.n_pad, a.eps2f, epi, st);
+    return a.newton ? launch_force_range<false, true>(tile, src, i0, nt, a.n_pad, a.eps2f, epi, st)
+                    : launch_force_range<false, false>(tile, src, i0, nt, a.n_pad, a.eps2f, epi, st);
+}
+
 hipError_t launch_fp32_accel(const StepArgs& a, const float4* src, bool compensated,
                              double* acc3, hipStream_t st)
 {

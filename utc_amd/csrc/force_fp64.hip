@@ -41,18 +41,22 @@ struct Epi64Step {
     }
 };
 
+// Field only; body i goes to row i - off (off = the first target of a slice).
 struct Epi64Accel {
     double* acc3;
+    int off = 0;
     __device__ __forceinline__ void operator()(int i, double ax, double ay, double az) const
     {
-        acc3[3 * (size_t)i + 0] = ax;
-        acc3[3 * (size_t)i + 1] = ay;
-        acc3[3 * (size_t)i + 2] = az;
+        const size_t r = 3 * (size_t)(i - off);
+        acc3[r + 0] = ax;
+        acc3[r + 1] = ay;
+        acc3[r + 2] = az;
     }
 };
 
+// Targets [i0, i0 + nt) against all n_pad sources.
 template <int S, int W, class Epi>
-__global__ __launch_bounds__(64 * W) void force64_kernel(const double4* __restrict__ src, int n,
+__global__ __launch_bounds__(64 * W) void force64_kernel(const double4* __restrict__ src, int i0, int nt,
                                                          int n_pad, double eps2, Epi epi)
 {
     constexpr int T = W / S;
@@ -60,8 +64,8 @@ __global__ __launch_bounds__(64 * W) void force64_kernel(const double4* __restri
     const int wave = threadIdx.x >> 6;
     const int tile = blockIdx.x * T + wave / S;
     const int part = wave % S;
-    const int i = tile * 64 + lane;
-    const double4 pi = src[i < n ? i : n - 1];
+    const int t = tile * 64 + lane;
+    const double4 pi = src[i0 + (t < nt ? t : nt - 1)];
     double hx = 0, hy = 0, hz = 0, cx = 0, cy = 0, cz = 0;
     const int len = n_pad / S;
     const int j0 = part * len;
@@ -120,46 +124,65 @@ __global__ __launch_bounds__(64 * W) void force64_kernel(const double4* __restri
             kahan_add64(hz, cz, red[wave + q][2][lane] - red[wave + q][5][lane]);
         }
     }
-    if (i < n)
-        epi(i, hx - cx, hy - cy, hz - cz);
+    if (t < nt)
+        epi(i0 + t, hx - cx, hy - cy, hz - cz);
 }
 
 template <int S, class Epi>
-static hipError_t launch64_tile(const double4* src, int n, int n_pad, double eps2, const Epi& epi,
+static hipError_t launch64_tile(const double4* src, int i0, int nt, int n_pad, double eps2, const Epi& epi,
                                 hipStream_t st)
 {
     constexpr int W = S < 4 ? 4 : S;
     constexpr int T = W / S;
-    const int tiles = (n + 63) / 64;
+    const int tiles = (nt + 63) / 64;
     const int blocks = (tiles + T - 1) / T;
-    hipLaunchKernelGGL((force64_kernel<S, W, Epi>), dim3(blocks), dim3(64 * W), 0, st, src, n,
+    hipLaunchKernelGGL((force64_kernel<S, W, Epi>), dim3(blocks), dim3(64 * W), 0, st, src, i0, nt,
                        n_pad, eps2, epi);
     return hipGetLastError();
 }
 
+// Source splits by the system's size n (enough waves to fill the chip for all
+// n targets); a slice keeps the split of the whole system, so its sums (and
+// results) are bitwise those of the same bodies in a full evaluation.
 template <class Epi>
-static hipError_t launch64(const double4* src, int n, int n_pad, double eps2, const Epi& epi,
+static hipError_t launch64(const double4* src, int i0, int nt, int n, int n_pad, double eps2, const Epi& epi,
                            hipStream_t st)
 {
+    if (nt < 1)
+        return hipErrorInvalidValue;
     const int waves = (n + 63) / 64;
     if (waves >= 4096)
-        return launch64_tile<4>(src, n, n_pad, eps2, epi, st);
+        return launch64_tile<4>(src, i0, nt, n_pad, eps2, epi, st);
     if (waves * 8 >= 4096 * 4)
-        return launch64_tile<8>(src, n, n_pad, eps2, epi, st);
-    return launch64_tile<16>(src, n, n_pad, eps2, epi, st);
+        return launch64_tile<8>(src, i0, nt, n_pad, eps2, epi, st);
+    return launch64_tile<16>(src, i0, nt, n_pad, eps2, epi, st);
 }
 
 hipError_t launch_fp64_step(const StepArgs& a, const double4* pos_old, double4* pos_new,
                             double4* vel, hipStream_t st)
 {
     Epi64Step epi{pos_old, pos_new, vel, a.dt, a.damping, a.kick};
-    return launch64(pos_old, a.n, a.n_pad, a.eps2, epi, st);
+    return launch64(pos_old, 0, a.n, a.n, a.n_pad, a.eps2, epi, st);
 }
 
 hipError_t launch_fp64_accel(const StepArgs& a, const double4* src, double* acc3, hipStream_t st)
 {
     Epi64Accel epi{acc3};
-    return launch64(src, a.n, a.n_pad, a.eps2, epi, st);
+    return launch64(src, 0, a.n, a.n, a.n_pad, a.eps2, epi, st);
+}
+
+hipError_t launch_fp64_step_slice(const StepArgs& a, int i0, int nt, const double4* pos_old, double4* pos_new,
+                                  double4* vel, hipStream_t st)
+{
+    Epi64Step epi{pos_old, pos_new, vel, a.dt, a.damping, a.kick};
+    return launch64(pos_old, i0, nt, a.n, a.n_pad, a.eps2, epi, st);
+}
+
+hipError_t launch_fp64_accel_slice(const StepArgs& a, int i0, int nt, const double4* src, double* acc3,
+                                   hipStream_t st)
+{
+    Epi64Accel epi{acc3, i0};
+    return launch64(src, i0, nt, a.n, a.n_pad, a.eps2, epi, st);
 }
 
 }  // namespace utcnb

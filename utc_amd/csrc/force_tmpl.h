@@ -158,24 +158,28 @@ struct EpiMixed {
     }
 };
 
-// Field only, as FP64 [3n].
+// Field only, as FP64 [3n]; body i goes to row i - off (off = the first
+// target of a slice, 0 for all bodies).
 struct EpiAccel {
     double* acc3;
+    int off = 0;
     __device__ __forceinline__ void operator()(int i, float hx, float hy, float hz, float cx,
                                                float cy, float cz) const
     {
-        acc3[3 * (size_t)i + 0] = (double)hx - (double)cx;
-        acc3[3 * (size_t)i + 1] = (double)hy - (double)cy;
-        acc3[3 * (size_t)i + 2] = (double)hz - (double)cz;
+        const size_t r = 3 * (size_t)(i - off);
+        acc3[r + 0] = (double)hx - (double)cx;
+        acc3[r + 1] = (double)hy - (double)cy;
+        acc3[r + 2] = (double)hz - (double)cz;
     }
 };
 
 // ---- the kernel --------------------------------------------------------------
 // W waves per workgroup; S of them split the sources of one target tile, so a
 // workgroup holds W/S target tiles of 64*K bodies.  `src` holds n_pad bodies
-// (multiple of kPadJ); targets are the first n.
+// (multiple of kPadJ); the targets are bodies [i0, i0 + nt) (all n bodies, or
+// one slice of them, against every source).
 template <int K, int S, int W, bool COMP, bool NEWTON, class Epi>
-__global__ __launch_bounds__(64 * W) void force_kernel(const float4* __restrict__ src, int n,
+__global__ __launch_bounds__(64 * W) void force_kernel(const float4* __restrict__ src, int i0, int nt,
                                                        int n_pad, float eps2, Epi epi)
 {
     constexpr int T = W / S;  // target tiles per workgroup
@@ -188,8 +192,8 @@ __global__ __launch_bounds__(64 * W) void force_kernel(const float4* __restrict_
     float hx[K], hy[K], hz[K], cx[K], cy[K], cz[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        int i = tile * 64 * K + k * 64 + lane;
-        int il = i < n ? i : n - 1;  // clamp: lanes past n compute but never store
+        int t = tile * 64 * K + k * 64 + lane;
+        int il = i0 + (t < nt ? t : nt - 1);  // clamp: lanes past the last target compute but never store
         float4 p = src[il];
         px[k] = p.x; py[k] = p.y; pz[k] = p.z;
         hx[k] = hy[k] = hz[k] = 0.f;
@@ -234,43 +238,53 @@ __global__ __launch_bounds__(64 * W) void force_kernel(const float4* __restrict_
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        int i = tile * 64 * K + k * 64 + lane;
-        if (i < n)
-            epi(i, hx[k], hy[k], hz[k], cx[k], cy[k], cz[k]);
+        int t = tile * 64 * K + k * 64 + lane;
+        if (t < nt)
+            epi(i0 + t, hx[k], hy[k], hz[k], cx[k], cy[k], cz[k]);
     }
 }
 
 template <int K, int S, bool COMP, bool NEWTON, class Epi>
-hipError_t launch_force_tile(const float4* src, int n, int n_pad, float eps2, const Epi& epi,
+hipError_t launch_force_tile(const float4* src, int i0, int nt, int n_pad, float eps2, const Epi& epi,
                              hipStream_t st)
 {
     constexpr int W = S < 4 ? 4 : S;
     constexpr int T = W / S;
-    const int tiles = (n + 64 * K - 1) / (64 * K);
+    const int tiles = (nt + 64 * K - 1) / (64 * K);
     const int blocks = (tiles + T - 1) / T;
     hipLaunchKernelGGL((force_kernel<K, S, W, COMP, NEWTON, Epi>), dim3(blocks), dim3(64 * W), 0,
-                       st, src, n, n_pad, eps2, epi);
+                       st, src, i0, nt, n_pad, eps2, epi);
     return hipGetLastError();
 }
 
-// Dispatch over the tile table (must match tile_get()).
+// Dispatch over the tile table (must match tile_get()); targets [i0, i0 + nt).
+template <bool COMP, bool NEWTON, class Epi>
+hipError_t launch_force_range(int tile, const float4* src, int i0, int nt, int n_pad, float eps2,
+                              const Epi& epi, hipStream_t st)
+{
+    if (nt < 1)
+        return hipErrorInvalidValue;
+    switch (tile) {
+    case 0: return launch_force_tile<1, 1, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 1: return launch_force_tile<2, 1, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 2: return launch_force_tile<4, 1, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 3: return launch_force_tile<1, 2, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 4: return launch_force_tile<2, 2, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 5: return launch_force_tile<1, 4, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 6: return launch_force_tile<2, 4, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 7: return launch_force_tile<1, 8, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 8: return launch_force_tile<2, 8, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 9: return launch_force_tile<1, 16, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// All n bodies as targets.
 template <bool COMP, bool NEWTON, class Epi>
 hipError_t launch_force(int tile, const float4* src, int n, int n_pad, float eps2,
                         const Epi& epi, hipStream_t st)
 {
-    switch (tile) {
-    case 0: return launch_force_tile<1, 1, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    case 1: return launch_force_tile<2, 1, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    case 2: return launch_force_tile<4, 1, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    case 3: return launch_force_tile<1, 2, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    case 4: return launch_force_tile<2, 2, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    case 5: return launch_force_tile<1, 4, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    case 6: return launch_force_tile<2, 4, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    case 7: return launch_force_tile<1, 8, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    case 8: return launch_force_tile<2, 8, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    case 9: return launch_force_tile<1, 16, COMP, NEWTON>(src, n, n_pad, eps2, epi, st);
-    default: return hipErrorInvalidValue;
-    }
+    return launch_force_range<COMP, NEWTON>(tile, src, 0, n, n_pad, eps2, epi, st);
 }
 
 }  // namespace utcnb

@@ -93,6 +93,32 @@ hipError_t launch_ref_accel_f32(const float4* src, int n, float eps2, double* ac
 hipError_t launch_ref_accel_f64(const double4* src, int n, double eps2, double* acc3,
                                 hipStream_t st);
 
+// ---- target slices: bodies [i0, i0 + nt) against all n sources ---------------
+// Reference arithmetic: pos_new / vel are indexed by the slice (t = i - i0);
+// the caller offsets full-size arrays.  acc3 rows are slice rows.
+hipError_t launch_ref_step_slice_f32(const float4* pos_old, float4* pos_new, float4* vel, int n, int i0, int nt,
+                                     float dt, float eps2, float damping, int block, hipStream_t st);
+hipError_t launch_ref_step_slice_f64(const double4* pos_old, double4* pos_new, double4* vel, int n, int i0,
+                                     int nt, double dt, double eps2, double damping, int block, hipStream_t st);
+hipError_t launch_ref_accel_slice_f32(const float4* src, int n, int i0, int nt, float eps2, double* acc3,
+                                      hipStream_t st);
+hipError_t launch_ref_accel_slice_f64(const double4* src, int n, int i0, int nt, double eps2, double* acc3,
+                                      hipStream_t st);
+// Fast paths on one-pass tiles (`tile`; the symmetric ones do not split by
+// target): full-size state arrays, updated for the slice only.
+int slice_tile_default(int64_t nt, int cu_count);
+hipError_t launch_fp32_step_slice(const StepArgs& a, int tile, int i0, int nt, const float4* pos_old,
+                                  float4* pos_new, float4* vel, hipStream_t st);
+hipError_t launch_mixed_step_slice(const StepArgs& a, int tile, int i0, int nt, const float4* snap_old,
+                                   float4* snap_new, double4* pos, double4* vel, hipStream_t st);
+hipError_t launch_fp32_accel_slice(const StepArgs& a, int tile, int i0, int nt, const float4* src,
+                                   bool compensated, double* acc3, hipStream_t st);
+// FP64: the source split of the whole system, so slices are bitwise the full field's rows.
+hipError_t launch_fp64_step_slice(const StepArgs& a, int i0, int nt, const double4* pos_old, double4* pos_new,
+                                  double4* vel, hipStream_t st);
+hipError_t launch_fp64_accel_slice(const StepArgs& a, int i0, int nt, const double4* src, double* acc3,
+                                   hipStream_t st);
+
 // ---- adjoint (adjoint.hip) ------------------------------------------------
 // jrec: 2 records per body {x,y,z,m}{ax,ay,az,0}, n_pad bodies.
 hipError_t launch_vjp_f32(const float4* jrec, int n, int n_pad, double eps2, double* xbar3,

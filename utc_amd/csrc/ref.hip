@@ -151,19 +151,24 @@ __device__ __forceinline__ void ref_field(const T4* __restrict__ pos, int n, T e
 }
 
 // Fused force + integrate; the twin of the reference's
-// _integrateNBodySystem_kernel<T> (benchapps/nbody/cuda/bodysystem_kernel.cu:87-138).
+// _integrateNBodySystem_kernel<T> (benchapps/nbody/cuda/bodysystem_kernel.cu:87-138)
+// and of its multi-GPU form that takes a slice of targets
+// (benchapps/nbody/utc-snode-mgpu/mgpu/bodysystem_kernel.cu:88-140): targets
+// i0 + t for t < cnt against all n sources of pos_old; the new position and
+// velocity of target i0 + t are pos_new[t] and vel[t] (the caller offsets
+// full-size arrays by i0).
 template <typename T, typename T4, bool FAST>
 __global__ void ref_step_kernel(const T4* __restrict__ pos_old, T4* __restrict__ pos_new,
-                                T4* __restrict__ vel, int n, T dt, T eps2, T damping)
+                                T4* __restrict__ vel, int n, int i0, int cnt, T dt, T eps2, T damping)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt)
         return;
-    T4 p = pos_old[i];
+    T4 p = pos_old[i0 + t];
     T a0, a1, a2;
     ref_field<T, T4, FAST>(pos_old, n, eps2, p, a0, a1, a2);
     T w = p.w;  // the reference's "invMass" (holds m_i), seq/bodysystem.cc:149
-    T4 v = vel[i];
+    T4 v = vel[t];
     v.x += (a0 * w) * dt;
     v.y += (a1 * w) * dt;
     v.z += (a2 * w) * dt;
@@ -173,75 +178,104 @@ __global__ void ref_step_kernel(const T4* __restrict__ pos_old, T4* __restrict__
     p.x += v.x * dt;
     p.y += v.y * dt;
     p.z += v.z * dt;
-    pos_new[i] = p;
-    vel[i] = v;
+    pos_new[t] = p;
+    vel[t] = v;
 }
 
+// Field of targets i0 + t, t < cnt, into acc3[3t .. 3t+2].
 template <typename T, typename T4, bool FAST>
-__global__ void ref_accel_kernel(const T4* __restrict__ pos, int n, T eps2,
+__global__ void ref_accel_kernel(const T4* __restrict__ pos, int n, int i0, int cnt, T eps2,
                                  double* __restrict__ acc3)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt)
         return;
     T a0, a1, a2;
-    ref_field<T, T4, FAST>(pos, n, eps2, pos[i], a0, a1, a2);
-    acc3[3 * (size_t)i + 0] = (double)a0;
-    acc3[3 * (size_t)i + 1] = (double)a1;
-    acc3[3 * (size_t)i + 2] = (double)a2;
+    ref_field<T, T4, FAST>(pos, n, eps2, pos[i0 + t], a0, a1, a2);
+    acc3[3 * (size_t)t + 0] = (double)a0;
+    acc3[3 * (size_t)t + 1] = (double)a1;
+    acc3[3 * (size_t)t + 2] = (double)a2;
+}
+
+hipError_t launch_ref_step_slice_f32(const float4* pos_old, float4* pos_new, float4* vel, int n, int i0, int cnt,
+                                     float dt, float eps2, float damping, int block, hipStream_t st)
+{
+    if (block <= 0)
+        block = 256;
+    const dim3 grid((unsigned)((cnt + block - 1) / block));
+    if (eps2 >= kRefInvMin && eps2 <= 3.0e38f)
+        hipLaunchKernelGGL((ref_step_kernel<float, float4, true>), grid, dim3(block), 0, st, pos_old, pos_new,
+                           vel, n, i0, cnt, dt, eps2, damping);
+    else
+        hipLaunchKernelGGL((ref_step_kernel<float, float4, false>), grid, dim3(block), 0, st, pos_old, pos_new,
+                           vel, n, i0, cnt, dt, eps2, damping);
+    return hipGetLastError();
+}
+
+hipError_t launch_ref_step_slice_f64(const double4* pos_old, double4* pos_new, double4* vel, int n, int i0,
+                                     int cnt, double dt, double eps2, double damping, int block, hipStream_t st)
+{
+    if (block <= 0)
+        block = 256;
+    const dim3 grid((unsigned)((cnt + block - 1) / block));
+    if (eps2 >= kRefInv64Min)
+        hipLaunchKernelGGL((ref_step_kernel<double, double4, true>), grid, dim3(block), 0, st, pos_old, pos_new,
+                           vel, n, i0, cnt, dt, eps2, damping);
+    else
+        hipLaunchKernelGGL((ref_step_kernel<double, double4, false>), grid, dim3(block), 0, st, pos_old, pos_new,
+                           vel, n, i0, cnt, dt, eps2, damping);
+    return hipGetLastError();
 }
 
 hipError_t launch_ref_step_f32(const float4* pos_old, float4* pos_new, float4* vel, int n,
                                float dt, float eps2, float damping, int block, hipStream_t st)
 {
-    if (block <= 0)
-        block = 256;
-    if (eps2 >= kRefInvMin && eps2 <= 3.0e38f)
-        hipLaunchKernelGGL((ref_step_kernel<float, float4, true>), dim3((n + block - 1) / block),
-                           dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
-    else
-        hipLaunchKernelGGL((ref_step_kernel<float, float4, false>), dim3((n + block - 1) / block),
-                           dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
-    return hipGetLastError();
+    return launch_ref_step_slice_f32(pos_old, pos_new, vel, n, 0, n, dt, eps2, damping, block, st);
 }
 
 hipError_t launch_ref_step_f64(const double4* pos_old, double4* pos_new, double4* vel, int n,
                                double dt, double eps2, double damping, int block,
                                hipStream_t st)
 {
-    if (block <= 0)
-        block = 256;
-    if (eps2 >= kRefInv64Min)
-        hipLaunchKernelGGL((ref_step_kernel<double, double4, true>), dim3((n + block - 1) / block),
-                           dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
+    return launch_ref_step_slice_f64(pos_old, pos_new, vel, n, 0, n, dt, eps2, damping, block, st);
+}
+
+hipError_t launch_ref_accel_slice_f32(const float4* src, int n, int i0, int cnt, float eps2, double* acc3,
+                                      hipStream_t st)
+{
+    const dim3 grid((unsigned)((cnt + 255) / 256));
+    if (eps2 >= kRefInvMin && eps2 <= 3.0e38f)
+        hipLaunchKernelGGL((ref_accel_kernel<float, float4, true>), grid, dim3(256), 0, st, src, n, i0, cnt, eps2,
+                           acc3);
     else
-        hipLaunchKernelGGL((ref_step_kernel<double, double4, false>), dim3((n + block - 1) / block),
-                           dim3(block), 0, st, pos_old, pos_new, vel, n, dt, eps2, damping);
+        hipLaunchKernelGGL((ref_accel_kernel<float, float4, false>), grid, dim3(256), 0, st, src, n, i0, cnt, eps2,
+                           acc3);
+    return hipGetLastError();
+}
+
+hipError_t launch_ref_accel_slice_f64(const double4* src, int n, int i0, int cnt, double eps2, double* acc3,
+                                      hipStream_t st)
+{
+    const dim3 grid((unsigned)((cnt + 255) / 256));
+    if (eps2 >= kRefInv64Min)
+        hipLaunchKernelGGL((ref_accel_kernel<double, double4, true>), grid, dim3(256), 0, st, src, n, i0, cnt,
+                           eps2, acc3);
+    else
+        hipLaunchKernelGGL((ref_accel_kernel<double, double4, false>), grid, dim3(256), 0, st, src, n, i0, cnt,
+                           eps2, acc3);
     return hipGetLastError();
 }
 
 hipError_t launch_ref_accel_f32(const float4* src, int n, float eps2, double* acc3,
                                 hipStream_t st)
 {
-    if (eps2 >= kRefInvMin && eps2 <= 3.0e38f)
-        hipLaunchKernelGGL((ref_accel_kernel<float, float4, true>), dim3((n + 255) / 256), dim3(256), 0,
-                           st, src, n, eps2, acc3);
-    else
-        hipLaunchKernelGGL((ref_accel_kernel<float, float4, false>), dim3((n + 255) / 256), dim3(256), 0,
-                           st, src, n, eps2, acc3);
-    return hipGetLastError();
+    return launch_ref_accel_slice_f32(src, n, 0, n, eps2, acc3, st);
 }
 
 hipError_t launch_ref_accel_f64(const double4* src, int n, double eps2, double* acc3,
                                 hipStream_t st)
 {
-    if (eps2 >= kRefInv64Min)
-        hipLaunchKernelGGL((ref_accel_kernel<double, double4, true>), dim3((n + 255) / 256), dim3(256), 0,
-                           st, src, n, eps2, acc3);
-    else
-        hipLaunchKernelGGL((ref_accel_kernel<double, double4, false>), dim3((n + 255) / 256), dim3(256),
-                           0, st, src, n, eps2, acc3);
-    return hipGetLastError();
+    return launch_ref_accel_slice_f64(src, n, 0, n, eps2, acc3, st);
 }
 
 }  // namespace utcnb
