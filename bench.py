@@ -333,6 +333,46 @@ def slice_per_gpu(U, c, n, K, shares=8):
         ms = (time.perf_counter() - t0) * 1e3 / K
         kms, _ = s.last_kernel_ms()
     best = min(sweep, key=sweep.get)
+
+    # the symmetric kernel's share of the same system: share 0 of `shares` by
+    # wave blocks, each unordered pair once (utcnb_share_pairs, then
+    # utcnb_share_finish with the reactions summed over all shares, computed
+    # once before the timed loop; the reduce-scatter and all-gather of a
+    # multi-GPU run are not part of one GPU's time)
+    import torch
+    sym = None
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+        s.set_state(c["pos"], c["vel"])
+        tile = s.tile()[1]
+        if tile.startswith("sym"):
+            reacts = [torch.empty((n, 3), dtype=torch.float64, device="cuda") for _ in range(shares)]
+            torch.cuda.synchronize()
+            for g in range(shares):
+                s.share_pairs(shares, g, reacts[g].data_ptr())
+            s.sync()
+            total = reacts[0].clone()
+            for g in range(1, shares):
+                total += reacts[g]
+            s0, c0 = s.share_range(shares, 0)
+            rsum = total[s0:s0 + c0].contiguous()
+            torch.cuda.synchronize()
+            for _ in range(2):  # warm-up
+                s.share_pairs(shares, 0, reacts[0].data_ptr())
+                s.share_finish(shares, 0, rsum.data_ptr())
+            s.sync()
+            t0 = time.perf_counter()
+            for _ in range(K):
+                s.share_pairs(shares, 0, reacts[0].data_ptr())
+                s.share_finish(shares, 0, rsum.data_ptr())
+            s.sync()
+            sms = (time.perf_counter() - t0) * 1e3 / K
+            sp = float(c0) * float(n)
+            sym = {"tile": tile, "shape": f"share 0 of {shares} by wave blocks: {c0} targets, each unordered pair "
+                                           "once against the half ring of wave blocks after its own",
+                   "targets": c0, "ms_per_step": sms, "interactions_per_s": sp / (sms * 1e-3),
+                   "fraction_of_fp32_peak": FLOP_PER_PAIR * sp / (sms * 1e-3) / PEAK_FP32}
+            log(f"symmetric share {c0}x{n} [{tile}]: {sms:.3f} ms/step, {sym['interactions_per_s']:.4e} "
+                f"interactions/s, {sym['fraction_of_fp32_peak']:.3f} of peak")
     out = {
         "shape": f"{cnt} targets x {n} sources (share 0 of {shares})", "targets": cnt, "sources": n,
         "default_tile": default, "ms_per_step": ms, "kernel_ms_last_step": kms,
@@ -341,6 +381,7 @@ def slice_per_gpu(U, c, n, K, shares=8):
         "best_tile": best, "best_ms_per_step": sweep[best],
         "best_fraction_of_fp32_peak": FLOP_PER_PAIR * pairs / (sweep[best] * 1e-3) / PEAK_FP32,
         "tile_sweep_ms": sweep,
+        "symmetric_share": sym,
     }
     log(f"slice {cnt}x{n} [{default}]: {ms:.3f} ms/step, {out['interactions_per_s']:.4e} interactions/s, "
         f"{out['fraction_of_fp32_peak']:.3f} of peak; best {best} {sweep[best]:.3f} ms")

@@ -177,6 +177,33 @@ int utcnb_commit_step(utcnb_system* sys);
  * body_task_mgpu.cu:109,134-140).
  */
 int utcnb_position_buffer(utcnb_system* sys, int which, void** pos4, int* dtype);
+/*
+ * The same share of a step for a MIXED system with a symmetric tile, which
+ * evaluates each unordered pair of bodies once: the shares are wave blocks
+ * (64*K bodies) rather than single bodies, and a pair of bodies owned by two
+ * shares is evaluated by one of them, which computes the reaction on the
+ * other's body.  A step of share `share` of `shares`:
+ *   1. utcnb_share_pairs: the pair work of the share's wave blocks; the
+ *      reactions it computes are summed per body into react3 (device FP64
+ *      [3n], written in full; the caller's buffer);
+ *   2. the caller sums react3 over the shares for the share's own bodies
+ *      (a reduce-scatter of the shares' react3 arrays in a multi-GPU run;
+ *      on one device, a plain sum);
+ *   3. utcnb_share_finish: the share's bodies [start, start + count)
+ *      (utcnb_share_range) add their target-side partials to react_sum3
+ *      (device FP64 [3*count]) and are integrated; their new positions go to
+ *      the next position buffer;
+ *   4. the caller gathers the other shares' positions into that buffer and
+ *      calls utcnb_commit_step.
+ * Every share's partial sums are those of utcnb_step with the same tile; only
+ * the FP64 order of the final per-body sums differs.  No reference
+ * counterpart (the reference's multi-GPU kernel evaluates every ordered pair,
+ * benchapps/nbody/utc-snode-mgpu/mgpu/bodysystem_kernel.cu:88-140).
+ * 1 <= shares <= the number of wave blocks; asynchronous on the handle's stream.
+ */
+int utcnb_share_range(const utcnb_system* sys, int shares, int share, int* start, int* count);
+int utcnb_share_pairs(utcnb_system* sys, int shares, int share, double* react3);
+int utcnb_share_finish(utcnb_system* sys, int shares, int share, const double* react_sum3);
 /* Field of bodies [start, start + count) as utcnb_step_slice computes it:
  * acc3 host FP64 [3 * count].  Synchronizes the stream. */
 int utcnb_accel_slice(utcnb_system* sys, int start, int count, double* acc3);

@@ -11,7 +11,12 @@ bodysystem_kernel.cu:88-140 slice kernel).
 * four handles on one GPU, each stepping its own slice and receiving the
   others' slices into its next position buffer (the all-gather of a sharded
   run), end bitwise where one handle's full steps end;
-* the slice field against the CPU FP64 oracle's rows.
+* the slice field against the CPU FP64 oracle's rows;
+* the shares of a symmetric-tile system (wave blocks, each unordered pair
+  once, reactions summed over the shares) follow the full symmetric step to
+  1e-12, and four handles running one share each with the reduce-scatter and
+  all-gather done between them end bitwise where one handle running every
+  share ends.
 """
 import ctypes as C
 import sys
@@ -223,3 +228,133 @@ def test_slice_field_against_cpu_fp64_oracle(which):
     assert errs["fp64"] <= 1e-14
     assert errs["mixed"] <= 2e-6
     assert errs["fp32"] <= 4e-6
+
+
+# ---- shares of a symmetric-tile system (each unordered pair once) -----------------
+def _share_step_all(s, G, n):
+    """One step of every share of one handle (the data flow of a G-GPU run
+    with the reduce-scatter and the all-gather done in place)."""
+    import torch
+    reacts = [torch.empty((n, 3), dtype=torch.float64, device="cuda") for _ in range(G)]
+    torch.cuda.synchronize()
+    for g in range(G):
+        s.share_pairs(G, g, reacts[g].data_ptr())
+    s.sync()
+    total = reacts[0].clone()
+    for g in range(1, G):
+        total += reacts[g]
+    torch.cuda.synchronize()
+    sums = []
+    for g in range(G):
+        start, cnt = s.share_range(G, g)
+        sums.append(total[start:start + cnt].contiguous())
+    torch.cuda.synchronize()
+    for g in range(G):
+        s.share_finish(G, g, sums[g].data_ptr())
+    s.sync()
+    s.commit_step()
+
+
+@pytest.mark.parametrize("n,G", [(65536, 8), (20001, 3)])
+def test_shares_equal_full_symmetric_steps(n, G):
+    """Every share of one handle, their reactions summed, then every share
+    finished and a commit: the full step's partial sums in another FP64 order,
+    so the trajectory matches utcnb_step with the same tile to 1e-12, and (at
+    the smaller N, where the CPU is quick) the FP64 CPU oracle to 1e-9 after
+    3 steps."""
+    c = plummer(n, eps=0.005, dt=5e-4)
+    steps = 3
+    say(f"case: shares vs full symmetric steps n={n} G={G}")
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as full:
+        assert full.tile()[1].startswith("sym")
+        full.set_state(c["pos"], c["vel"])
+        full.step(steps)
+        pf, vf = full.get_state()
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+        s.set_state(c["pos"], c["vel"])
+        ranges = [s.share_range(G, g) for g in range(G)]
+        assert ranges[0][0] == 0 and sum(cnt for _, cnt in ranges) == n
+        assert all(ranges[g][0] + ranges[g][1] == ranges[g + 1][0] for g in range(G - 1))
+        for _ in range(steps):
+            _share_step_all(s, G, n)
+        ps, vs = s.get_state()
+    err = relnorm(ps[:, :3], pf[:, :3])
+    say(f"  shares vs full step: position rel {err:.3e}")
+    assert err <= 1e-12
+    assert relnorm(vs[:, :3], vf[:, :3]) <= 1e-10
+    if n <= 32768:
+        p64, _ = O.steps_f64(c["pos"], c["vel"], steps, c["eps"] ** 2, c["dt"])
+        assert relnorm(ps[:, :3], p64[:, :3]) <= 1e-9
+
+
+def test_sharded_symmetric_handles_equal_one_handle():
+    """Four handles on one GPU stand for four GPUs: each runs its own share,
+    the shares' reaction arrays are summed (the reduce-scatter), each finishes
+    its share, and the new FP32 positions are gathered into every handle's
+    next buffer (the all-gather).  Bitwise the state of one handle running
+    every share itself."""
+    import torch
+    n, G, steps = 20001, 4, 3
+    c = plummer(n, eps=0.005, dt=5e-4)
+    hip = _hip()
+    say(f"case: {G} sharded symmetric handles n={n}")
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as one:
+        one.set_state(c["pos"], c["vel"])
+        for _ in range(steps):
+            _share_step_all(one, G, n)
+        ref_p, ref_v = one.get_state()
+    hs = [U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) for _ in range(G)]
+    try:
+        for h in hs:
+            h.set_state(c["pos"], c["vel"])
+        ranges = [hs[0].share_range(G, g) for g in range(G)]
+        reacts = [torch.empty((n, 3), dtype=torch.float64, device="cuda") for _ in range(G)]
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            for g, h in enumerate(hs):
+                h.share_pairs(G, g, reacts[g].data_ptr())
+            for h in hs:
+                h.sync()
+            total = reacts[0].clone()
+            for g in range(1, G):
+                total += reacts[g]
+            sums = [total[a:a + k].contiguous() for a, k in ranges]
+            torch.cuda.synchronize()
+            for g, h in enumerate(hs):
+                h.share_finish(G, g, sums[g].data_ptr())
+            for h in hs:
+                h.sync()
+            for g, h in enumerate(hs):
+                dst, _ = h.position_buffer(1)
+                for r, (a, k) in enumerate(ranges):
+                    if r != g:
+                        src, _ = hs[r].position_buffer(1)
+                        assert hip.hipMemcpy(dst + 16 * a, src + 16 * a, 16 * k, 3) == 0
+            assert hip.hipDeviceSynchronize() == 0
+            for h in hs:
+                h.commit_step()
+        got_p = np.zeros_like(ref_p)
+        got_v = np.zeros_like(ref_v)
+        for h, (a, k) in zip(hs, ranges):
+            p, v = h.get_state()
+            got_p[a:a + k] = p[a:a + k]
+            got_v[a:a + k] = v[a:a + k]
+    finally:
+        for h in hs:
+            h.close()
+    assert np.array_equal(got_p, ref_p)
+    assert np.array_equal(got_v, ref_v)
+
+
+def test_share_calls_reject_other_systems():
+    L = U.lib()
+    tid = U.tiles().index("k2s8") + 1
+    a, b = C.c_int(), C.c_int()
+    with U.System(4096, U.PREC_MIXED, 1e-4, 1e-3, tile=tid) as s:
+        assert L.utcnb_share_range(s.h, 2, 0, C.byref(a), C.byref(b)) == U.EINVAL
+        assert b"symmetric" in L.utcnb_last_error()
+    with U.System(4096, U.PREC_MIXED, 1e-4, 1e-3) as s:
+        assert s.tile()[1].startswith("sym")
+        assert L.utcnb_share_range(s.h, 0, 0, C.byref(a), C.byref(b)) == U.EINVAL
+        assert L.utcnb_share_range(s.h, 2, 2, C.byref(a), C.byref(b)) == U.EINVAL
+        assert L.utcnb_share_range(s.h, 100000, 0, C.byref(a), C.byref(b)) == U.EINVAL

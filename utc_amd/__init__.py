@@ -36,7 +36,8 @@ EXPORTS = [
     "utcnb_rng_seed", "utcnb_rng_next", "utcnb_splitmix64", "utcnb_generate",
     "utcnb_last_error", "utcnb_version", "utcnb_device_info", "utcnb_accel_noalloc",
     "utcnb_integrate_slice_f32", "utcnb_integrate_slice_f64", "utcnb_step_slice", "utcnb_commit_step",
-    "utcnb_position_buffer", "utcnb_accel_slice", "utcnb_slice_tile",
+    "utcnb_position_buffer", "utcnb_accel_slice", "utcnb_slice_tile", "utcnb_share_range",
+    "utcnb_share_pairs", "utcnb_share_finish",
 ]
 
 
@@ -106,6 +107,9 @@ def _proto(lib):
         "utcnb_position_buffer": (C.c_int, [_vp, C.c_int, P(_vp), P(C.c_int)]),
         "utcnb_accel_slice": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
         "utcnb_slice_tile": (C.c_int, [_vp, C.c_int, P(C.c_int)]),
+        "utcnb_share_range": (C.c_int, [_vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int)]),
+        "utcnb_share_pairs": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+        "utcnb_share_finish": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):
@@ -225,6 +229,18 @@ class System:
         a = np.zeros((count, 3), np.float64)
         check(lib().utcnb_accel_slice(self.h, start, count, ptr(a)), "utcnb_accel_slice")
         return a
+
+    def share_range(self, shares: int, share: int):
+        """(start, count) of the bodies of share `share` of `shares` (symmetric tiles)."""
+        a, b = C.c_int(), C.c_int()
+        check(lib().utcnb_share_range(self.h, shares, share, C.byref(a), C.byref(b)), "utcnb_share_range")
+        return a.value, b.value
+
+    def share_pairs(self, shares: int, share: int, react3_ptr: int):
+        check(lib().utcnb_share_pairs(self.h, shares, share, C.c_void_p(react3_ptr)), "utcnb_share_pairs")
+
+    def share_finish(self, shares: int, share: int, react_sum3_ptr: int):
+        check(lib().utcnb_share_finish(self.h, shares, share, C.c_void_p(react_sum3_ptr)), "utcnb_share_finish")
 
     def slice_tile(self, count: int):
         """(tile id, name) that the slice calls run for `count` targets."""

@@ -10,6 +10,7 @@
 //   FP64              : pos double4[2] padded + vel double4
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -51,6 +52,8 @@ struct utcnb_system {
     double* scratch = nullptr;   // 3n doubles (field) / energy partials
     float* ipart = nullptr;      // symmetric-tile workspace
     float* jpart = nullptr;
+    float* share_ipart = nullptr;  // target-side partials of a share step (utcnb_share_pairs)
+    size_t share_ipart_floats = 0;
     size_t scratch_bytes = 0;
     void* stage = nullptr;       // conversion staging, 4n * 8 bytes
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -173,6 +176,8 @@ void release(utcnb_system* s)
     dfree(s->scratch);
     dfree(s->ipart);
     dfree(s->jpart);
+    dfree(s->share_ipart);
+    s->share_ipart_floats = 0;
     if (s->stage)
         (void)hipFree(s->stage);
     s->stage = nullptr;
@@ -732,6 +737,98 @@ int utcnb_position_buffer(utcnb_system* s, int which, void** pos4, int* dtype)
     *pos4 = p;
     if (dtype)
         *dtype = dt;
+    return UTCNB_OK;
+}
+
+// ---- shares of a symmetric-tile system (wave-block partition) -------------------
+struct ShareGeom {
+    int NW, w0, w1, start, count, Rs;
+};
+
+static int share_geom(const utcnb_system* s, int shares, int share, const char* who, ShareGeom* g)
+{
+    if (s->cfg.precision != UTCNB_PREC_MIXED || !tile_get(s->args.tile).sym)
+        return fail(UTCNB_EINVAL, std::string(who) + ": needs a MIXED system with a symmetric tile "
+                                                     "(utcnb_step_slice shares any other system)");
+    const Tile& t = tile_get(s->args.tile);
+    const int B = 64 * t.k;
+    g->NW = s->n_pad / B;
+    if (shares < 1 || shares > g->NW || share < 0 || share >= shares)
+        return fail(UTCNB_EINVAL, std::string(who) + ": need 1 <= shares <= wave blocks (" +
+                                      std::to_string(g->NW) + ") and 0 <= share < shares");
+    sym_share_blocks(g->NW, shares, share, &g->w0, &g->w1);
+    g->start = std::min(g->w0 * B, s->n);
+    g->count = std::min(g->w1 * B, s->n) - g->start;
+    // one split count for every share of the partition (from the largest
+    // share), so the shares' target-side partials share one layout
+    g->Rs = sym_share_splits((g->NW + shares - 1) / shares, t.s, t.k, g->NW);
+    return UTCNB_OK;
+}
+
+int utcnb_share_range(const utcnb_system* s, int shares, int share, int* start, int* count)
+{
+    if (!s || !start || !count)
+        return fail(UTCNB_EINVAL, "utcnb_share_range: null argument");
+    ShareGeom g;
+    if (int rc = share_geom(s, shares, share, "utcnb_share_range", &g))
+        return rc;
+    *start = g.start;
+    *count = g.count;
+    return UTCNB_OK;
+}
+
+int utcnb_share_pairs(utcnb_system* s, int shares, int share, double* react3)
+{
+    if (!s || !react3)
+        return fail(UTCNB_EINVAL, "utcnb_share_pairs: null argument");
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_share_pairs: no state set");
+    ShareGeom g;
+    if (int rc = share_geom(s, shares, share, "utcnb_share_pairs", &g))
+        return rc;
+    const size_t need = (size_t)g.Rs * (size_t)s->n_pad * 3;
+    if (need > s->share_ipart_floats) {
+        if (need * sizeof(float) > s->partial_budget)
+            return fail(UTCNB_EINVAL, "utcnb_share_pairs: the share's partial sums exceed the device budget");
+        dfree(s->share_ipart);
+        s->share_ipart_floats = 0;
+        if (int rc = dalloc(&s->share_ipart, need))
+            return rc;
+        s->share_ipart_floats = need;
+    }
+    UTCNB_HIP(hipEventRecord(s->ev0, s->stream));
+    hipError_t e = launch_sym_share_pairs(tile_get(s->args.tile), s->args.newton, s->snap[s->cur], s->n, s->n_pad,
+                                          s->args.eps2f, shares, share, g.Rs, s->share_ipart, s->jpart, react3,
+                                          s->stream);
+    if (e != hipSuccess)
+        return fail(UTCNB_EHIP, std::string("utcnb_share_pairs: ") + hipGetErrorString(e));
+    UTCNB_HIP(hipEventRecord(s->ev1, s->stream));
+    s->last_launches = 1;
+    s->timed = true;
+    return UTCNB_OK;
+}
+
+int utcnb_share_finish(utcnb_system* s, int shares, int share, const double* react_sum3)
+{
+    if (!s || !react_sum3)
+        return fail(UTCNB_EINVAL, "utcnb_share_finish: null argument");
+    if (!s->has_state)
+        return fail(UTCNB_ESTATE, "utcnb_share_finish: no state set");
+    ShareGeom g;
+    if (int rc = share_geom(s, shares, share, "utcnb_share_finish", &g))
+        return rc;
+    if (!s->share_ipart || (size_t)g.Rs * (size_t)s->n_pad * 3 > s->share_ipart_floats)
+        return fail(UTCNB_ESTATE, "utcnb_share_finish: utcnb_share_pairs of this share has not run");
+    if (g.count <= 0)
+        return UTCNB_OK;
+    UTCNB_HIP(hipEventRecord(s->ev0, s->stream));
+    hipError_t e = launch_sym_share_finish_mixed(s->args, g.Rs, g.start, g.count, s->share_ipart, react_sum3,
+                                                 s->snap[1 - s->cur], s->p64[0], s->v64, s->stream);
+    if (e != hipSuccess)
+        return fail(UTCNB_EHIP, std::string("utcnb_share_finish: ") + hipGetErrorString(e));
+    UTCNB_HIP(hipEventRecord(s->ev1, s->stream));
+    s->last_launches = 1;
+    s->timed = true;
     return UTCNB_OK;
 }
 

@@ -119,6 +119,23 @@ hipError_t launch_fp64_step_slice(const StepArgs& a, int i0, int nt, const doubl
 hipError_t launch_fp64_accel_slice(const StepArgs& a, int i0, int nt, const double4* src, double* acc3,
                                    hipStream_t st);
 
+// ---- one share of a symmetric-tile system sharded by wave blocks (sym.hip) -----
+// Wave blocks [w0, w1) of share `share` of `shares` (the first NW % shares
+// shares take one block more).
+void sym_share_blocks(int NW, int shares, int share, int* w0, int* w1);
+// Waves per wave block for a share of nw_share blocks (R of the tile or more).
+int sym_share_splits(int nw_share, int R, int K, int NW);
+// The share's pair work (target-side partials into ipart [Rs][n_pad][3],
+// reactions into jpart) and its reactions summed per body into react3 (FP64 [3n]).
+hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
+                                  int shares, int share, int Rs, float* ipart, float* jpart, double* react3,
+                                  hipStream_t st);
+// Bodies [start, start + count): target-side partials + react_sum3 [3*count],
+// then the MIXED epilogue (FP64 state in place, new snapshot into snap_new).
+hipError_t launch_sym_share_finish_mixed(const StepArgs& a, int Rs, int start, int count, const float* ipart,
+                                         const double* react_sum3, float4* snap_new, double4* pos, double4* vel,
+                                         hipStream_t st);
+
 // ---- adjoint (adjoint.hip) ------------------------------------------------
 // jrec: 2 records per body {x,y,z,m}{ax,ay,az,0}, n_pad bodies.
 hipError_t launch_vjp_f32(const float4* jrec, int n, int n_pad, double eps2, double* xbar3,

@@ -449,6 +449,161 @@ hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, 
     return hipGetLastError();
 }
 
+// ---- one share of a system sharded over targets by wave blocks -----------------
+// Share g of G owns wave blocks [w0, w1) (the first NW % G shares one block
+// more) and evaluates, for each of them, the same sub-units as sym_wave does
+// in a full step: its own blocks and the half ring after it.  Its target-side
+// partials go to ipart (R_share rows, laid out as a full step's); its
+// reaction partials (for bodies of the wave blocks after each of its blocks,
+// mostly owned by other shares) are summed per destination body into react3,
+// which a sharded run adds up over the shares (a reduce-scatter); the share
+// then adds its bodies' target-side partials to the summed reactions and
+// integrates them.  On one device, running every share and summing the react3
+// arrays gives the full step's sums in another FP64 order.
+
+template <int K, bool NEWTON>
+__global__ __launch_bounds__(256) void sym_share_kernel(const float4* __restrict__ src, int NW, int R, int w0,
+                                                        int nwaves, float eps2, float* __restrict__ ipart,
+                                                        float* __restrict__ jpart)
+{
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (gw >= nwaves)
+        return;  // whole waves only
+    sym_wave<K, NEWTON, true>(src, NW, R, w0 + gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
+}
+
+// react3[b] (FP64, bodies b < n) = the reactions on body b from the wave
+// blocks [w0, w1), in increasing block order; 0 where none of them reaches b.
+__global__ __launch_bounds__(256) void sym_react_kernel(const float* __restrict__ jpart, int n, int NW, int K,
+                                                        int w0, int w1, double* __restrict__ react3)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n)
+        return;
+    const int half = NW / 2, B = 64 * K;
+    const int v = b / B, loc = b - v * B;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int w = w0; w < w1; ++w) {
+        int d = v - 1 - w;
+        if (d < 0)
+            d += NW;
+        if (d >= half || (2 * (d + 1) == NW && w >= half))
+            continue;  // v is not in w's half ring (or w's upper half skipped it)
+        const float* p = jpart + (((size_t)w * half + d) * B + loc) * 3;
+        s0 += p[0];
+        s1 += p[1];
+        s2 += p[2];
+    }
+    react3[3 * (size_t)b + 0] = s0;
+    react3[3 * (size_t)b + 1] = s1;
+    react3[3 * (size_t)b + 2] = s2;
+}
+
+// Bodies [start, start + count): their R target-side partials summed in FP64
+// by T lanes per body (lane t takes rows t, t + T, ...; a fixed butterfly
+// combines the lanes, as in sym_reduce.h), plus react_sum3[j] (the reactions
+// summed over the shares), then the epilogue.
+template <int T, class Epi>
+__global__ __launch_bounds__(256) void sym_share_finish_kernel(const float* __restrict__ ipart, int n_pad, int R,
+                                                               int start, int count,
+                                                               const double* __restrict__ react_sum3, Epi epi)
+{
+    int j, t;
+    sym_reduce_lane<T>(j, t);
+    const int b = start + (j < count ? j : count - 1);
+    double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int r = t; r < R; r += T)
+        part_add(s, ((const Part3*)ipart)[(size_t)r * n_pad + b]);
+#pragma unroll
+    for (int off = 32; off >= 64 / T; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            s[c] += __shfl_xor(s[c], off, 64);
+    if (j < count && t == 0) {
+        s[0] += react_sum3[3 * (size_t)j + 0];
+        s[1] += react_sum3[3 * (size_t)j + 1];
+        s[2] += react_sum3[3 * (size_t)j + 2];
+        sym_epilogue(epi, b, s);
+    }
+}
+
+void sym_share_blocks(int NW, int shares, int share, int* w0, int* w1)
+{
+    const int per = NW / shares, extra = NW % shares;
+    *w0 = share < extra ? share * (per + 1) : share * per + extra;
+    *w1 = *w0 + (share < extra ? per + 1 : per);
+}
+
+int sym_share_splits(int nw_share, int R, int K, int NW)
+{
+    // about 32768 waves (ten rounds of the chip's 256 CUs x 4 SIMDs x 3 waves
+    // of this kernel), every wave with the same number u of the Q sub-units
+    // of its wave block (the last one of a block fewer): u = Q nw / 32768,
+    // R_share = ceil(Q / u)
+    (void)R;
+    const long long Q = (long long)K * (NW / 2 + 1);
+    long long u = Q * nw_share / 32768;
+    if (u < 1)
+        u = 1;
+    return (int)((Q + u - 1) / u);
+}
+
+hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
+                                  int shares, int share, int Rs, float* ipart, float* jpart, double* react3,
+                                  hipStream_t st)
+{
+    const int K = t.k;
+    if (n_pad % (128 * K) != 0 || !ipart || !jpart)
+        return hipErrorInvalidValue;
+    const int NW = n_pad / (64 * K);
+    int w0, w1;
+    sym_share_blocks(NW, shares, share, &w0, &w1);
+    const int nwaves = (w1 - w0) * Rs;
+    if (nwaves > 0) {
+        const dim3 grid((nwaves + 3) / 4), block(256);
+#define SHARE_LAUNCH(KK)                                                                                  \
+    if (newton)                                                                                           \
+        hipLaunchKernelGGL((sym_share_kernel<KK, true>), grid, block, 0, st, src, NW, Rs, w0, nwaves, eps2, \
+                           ipart, jpart);                                                                 \
+    else                                                                                                  \
+        hipLaunchKernelGGL((sym_share_kernel<KK, false>), grid, block, 0, st, src, NW, Rs, w0, nwaves, eps2, \
+                           ipart, jpart);
+        if (K == 2) {
+            SHARE_LAUNCH(2)
+        } else if (K == 4) {
+            SHARE_LAUNCH(4)
+        } else if (K == 8) {
+            SHARE_LAUNCH(8)
+        } else {
+            return hipErrorInvalidValue;
+        }
+#undef SHARE_LAUNCH
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    hipLaunchKernelGGL(sym_react_kernel, dim3((n + 255) / 256), dim3(256), 0, st, jpart, n, NW, K, w0, w1, react3);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_share_finish_mixed(const StepArgs& a, int Rs, int start, int count, const float* ipart,
+                                         const double* react_sum3, float4* snap_new, double4* pos, double4* vel,
+                                         hipStream_t st)
+{
+    EpiMixed epi{pos, vel, snap_new, a.dt, a.damping, a.kick};
+    const int T = sym_reduce_lanes(count);
+    const dim3 grid((unsigned)(((long long)count * T + 255) / 256)), block(256);
+    switch (T) {
+    case 1: hipLaunchKernelGGL((sym_share_finish_kernel<1, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
+    case 2: hipLaunchKernelGGL((sym_share_finish_kernel<2, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
+    case 4: hipLaunchKernelGGL((sym_share_finish_kernel<4, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
+    case 8: hipLaunchKernelGGL((sym_share_finish_kernel<8, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
+    default: hipLaunchKernelGGL((sym_share_finish_kernel<16, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_counter_reset(int* counter, hipStream_t st)
 {
     hipLaunchKernelGGL(batch_counter_reset_kernel, dim3(1), dim3(64), 0, st, counter);
