@@ -388,6 +388,29 @@ def slice_per_gpu(U, c, n, K, shares=8):
     return out
 
 
+def large_n(U, n=2_500_000, steps=2):
+    """A system whose whole-step partial sums exceed the device budget (N =
+    2.5M Plummer, eps = 0.002, dt = 2.5e-4, mixed): the default keeps the
+    symmetric tile and evaluates each step in chunks of wave blocks."""
+    from tests.nbcases import plummer
+    c = plummer(n, eps=0.002, dt=2.5e-4)
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+        s.set_state(c["pos"], c["vel"])
+        s.step(1)  # warm-up
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        pairs = float(n) * float(n)
+        out = {"n": n, "tile": s.tile()[1], "chunks": s.sym_chunks(), "ms_per_step": ms,
+               "interactions_per_s": pairs / (ms * 1e-3),
+               "fraction_of_fp32_peak": FLOP_PER_PAIR * pairs / (ms * 1e-3) / PEAK_FP32}
+    log(f"N={n} [{out['tile']}, {out['chunks']} chunks]: {ms:.1f} ms/step, {out['interactions_per_s']:.4e} "
+        f"interactions/s, {out['fraction_of_fp32_peak']:.3f} of peak")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -399,7 +422,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=262144,
                     help="rows of the CPU baseline's force evaluation (default: all, ~10 s on 16 threads)")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched", "reference", "slice"], default=None,
+    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched", "reference", "slice", "large"],
+                    default=None,
                     help="time one path only and print its line (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 256 MiB device copy (to scale HBM counters) and exit")
@@ -449,6 +473,11 @@ def main():
         print(json.dumps({"metric": METRIC, "value": r["interactions_per_s"], "unit": "interactions/s",
                           "n_gpus": 1, "steps": K, "only": "reference", "reference_arithmetic": r}),
               flush=True)
+        return
+    if args.only == "large":
+        r = large_n(U)
+        print(json.dumps({"metric": METRIC, "value": r["interactions_per_s"], "unit": "interactions/s",
+                          "n_gpus": 1, "only": "large", "large_n": r}), flush=True)
         return
     if args.only == "slice":
         r = slice_per_gpu(U, c, n, K)
@@ -578,6 +607,13 @@ def main():
         slc = {"error": str(e)}
         log(f"slice leg failed: {e}")
 
+    big = None
+    try:
+        big = large_n(U)
+    except Exception as e:  # reported, never hidden
+        big = {"error": str(e)}
+        log(f"large-N leg failed: {e}")
+
     # ---- 256 batched systems -------------------------------------------------------
     batch = None
     if not args.no_batch:
@@ -680,6 +716,7 @@ def main():
         "reference_arithmetic": ref_arith,
         "batched": batch,
         "slice_per_gpu": slc,
+        "large_n": big,
         "fraction_of_fp32_peak": achieved / PEAK_FP32,
         "roofline": roofline,
         "cpu_baseline": cpu,

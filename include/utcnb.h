@@ -58,6 +58,11 @@ extern "C" {
 /* config flags */
 #define UTCNB_FLAG_NO_NEWTON 1 /* FP32/MIXED: use v_rsq_f32 without the Newton step
                                   (same measured accuracy, ~20% faster) */
+#define UTCNB_FLAG_SYM_CHUNKS 2 /* MIXED with a symmetric tile: evaluate each step in
+                                   chunks of wave blocks (partial-sum memory for one
+                                   chunk's reactions, one FP64 pass more); taken
+                                   without the flag when the whole step's partial
+                                   sums exceed the device budget */
 
 /* ---- kernel-level entry points (device pointers, reference layout) ------ */
 /*
@@ -257,6 +262,12 @@ int utcnb_adjoint(utcnb_system* sys, int nsteps, int ckpt_every, const double* t
  * utcnb_adjoint call on this handle, measured with HIP events on its stream;
  * *launches receives their count.  Synchronizes the stream. */
 int utcnb_last_kernel_ms(utcnb_system* sys, double* ms, int* launches);
+
+/* Chunks of wave blocks a MIXED handle's symmetric steps are evaluated in
+ * (UTCNB_FLAG_SYM_CHUNKS, or a system whose partial sums exceed the budget in
+ * one launch); 0 for a single launch or no symmetric tile.  No reference
+ * counterpart. */
+int utcnb_sym_chunks(const utcnb_system* sys, int* chunks);
 
 /* Force-kernel tile id (1..utcnb_tile_count()) the handle runs: the configured
  * one, or the built-in default for its size when utcnb_config.tile was 0; 0

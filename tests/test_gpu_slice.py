@@ -358,3 +358,62 @@ def test_share_calls_reject_other_systems():
         assert L.utcnb_share_range(s.h, 0, 0, C.byref(a), C.byref(b)) == U.EINVAL
         assert L.utcnb_share_range(s.h, 2, 2, C.byref(a), C.byref(b)) == U.EINVAL
         assert L.utcnb_share_range(s.h, 100000, 0, C.byref(a), C.byref(b)) == U.EINVAL
+
+
+# ---- the symmetric step in chunks of wave blocks (less partial-sum memory) ---------
+def test_chunked_symmetric_step_equals_two_shares():
+    """UTCNB_FLAG_SYM_CHUNKS at N = 20001: the step runs as two chunks of wave
+    blocks, one after the other, the reactions accumulated in FP64; bitwise the
+    two-share flow on one handle, and within 1e-12 of the single-launch
+    symmetric step; the field within 1e-7 of the single launch's."""
+    n, steps = 20001, 3
+    c = plummer(n, eps=0.005, dt=5e-4)
+    say(f"case: chunked symmetric step n={n}")
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON | U.FLAG_SYM_CHUNKS) as ch:
+        assert ch.sym_chunks() == 2
+        ch.set_state(c["pos"], c["vel"])
+        a_ch = ch.accel()
+        ch.step(steps)
+        pc, vc = ch.get_state()
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as sh:
+        assert sh.sym_chunks() == 0
+        sh.set_state(c["pos"], c["vel"])
+        a_full = sh.accel()
+        for _ in range(steps):
+            _share_step_all(sh, 2, n)
+        ps, vs = sh.get_state()
+    assert np.array_equal(pc, ps)
+    assert np.array_equal(vc, vs)
+    assert relnorm(a_ch, a_full) <= 1e-7
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as full:
+        full.set_state(c["pos"], c["vel"])
+        full.step(steps)
+        pf, _ = full.get_state()
+    assert relnorm(pc[:, :3], pf[:, :3]) <= 1e-12
+
+
+def test_large_n_takes_chunked_symmetric_steps():
+    """N = 2.5M: the whole step's reaction partials (73 GB) exceed the device
+    budget, so the default MIXED system keeps the symmetric tile and evaluates
+    it in chunks; its field at 512 sampled bodies against the CPU FP64 oracle,
+    and one step against the field's kick-drift."""
+    n = 2_500_000
+    c = plummer(n, eps=0.002, dt=2.5e-4)
+    rows = np.sort(np.random.default_rng(5).choice(n, 512, replace=False))
+    say(f"case: large N={n} chunked symmetric default")
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON) as s:
+        name, G = s.tile()[1], s.sym_chunks()
+        say(f"  tile {name}, {G} chunks")
+        assert name.startswith("sym") and G >= 2
+        s.set_state(c["pos"], c["vel"])
+        a = s.accel()
+        s.step(1)
+        p1, v1 = s.get_state()
+    ref = O.accel_rows_f64(c["pos"], c["eps"] ** 2, rows, nthreads=16)
+    err = relnorm(a[rows], ref)
+    say(f"  field rel err vs CPU FP64 oracle: {err:.3e}")
+    assert err <= 2e-6
+    # physical kick: v1 = v0 + a dt, x1 = x0 + v1 dt (FP64 state, the field above)
+    v_want = c["vel"][:, :3] + a * c["dt"]
+    assert relnorm(v1[:, :3], v_want) <= 1e-12
+    assert np.all(np.isfinite(p1))

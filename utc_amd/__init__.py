@@ -24,6 +24,7 @@ KICK_REFERENCE, KICK_PHYSICAL = 0, 1
 F32, F64 = 0, 1
 HOST, DEVICE = 0, 1
 FLAG_NO_NEWTON = 1
+FLAG_SYM_CHUNKS = 2
 GEN_CUBE, GEN_CUBE_LOGM, GEN_PLUMMER, GEN_TARGETS = 0, 1, 2, 3
 
 # Every symbol the header declares (checked by the CPU tests).
@@ -37,7 +38,7 @@ EXPORTS = [
     "utcnb_last_error", "utcnb_version", "utcnb_device_info", "utcnb_accel_noalloc",
     "utcnb_integrate_slice_f32", "utcnb_integrate_slice_f64", "utcnb_step_slice", "utcnb_commit_step",
     "utcnb_position_buffer", "utcnb_accel_slice", "utcnb_slice_tile", "utcnb_share_range",
-    "utcnb_share_pairs", "utcnb_share_finish",
+    "utcnb_share_pairs", "utcnb_share_finish", "utcnb_sym_chunks",
 ]
 
 
@@ -110,6 +111,7 @@ def _proto(lib):
         "utcnb_share_range": (C.c_int, [_vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int)]),
         "utcnb_share_pairs": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
         "utcnb_share_finish": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+        "utcnb_sym_chunks": (C.c_int, [_vp, P(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):
@@ -241,6 +243,12 @@ class System:
 
     def share_finish(self, shares: int, share: int, react_sum3_ptr: int):
         check(lib().utcnb_share_finish(self.h, shares, share, C.c_void_p(react_sum3_ptr)), "utcnb_share_finish")
+
+    def sym_chunks(self):
+        """Chunks of wave blocks the symmetric steps run in (0: one launch)."""
+        g = C.c_int()
+        check(lib().utcnb_sym_chunks(self.h, C.byref(g)), "utcnb_sym_chunks")
+        return g.value
 
     def slice_tile(self, count: int):
         """(tile id, name) that the slice calls run for `count` targets."""

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -54,6 +55,9 @@ struct utcnb_system {
     float* jpart = nullptr;
     float* share_ipart = nullptr;  // target-side partials of a share step (utcnb_share_pairs)
     size_t share_ipart_floats = 0;
+    float* cipart = nullptr;     // chunked symmetric steps: target-side partials,
+    float* cjpart = nullptr;     //   one chunk's reaction partials,
+    double* react = nullptr;     //   reactions summed per body
     size_t scratch_bytes = 0;
     void* stage = nullptr;       // conversion staging, 4n * 8 bytes
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -178,6 +182,9 @@ void release(utcnb_system* s)
     dfree(s->jpart);
     dfree(s->share_ipart);
     s->share_ipart_floats = 0;
+    dfree(s->cipart);
+    dfree(s->cjpart);
+    dfree(s->react);
     if (s->stage)
         (void)hipFree(s->stage);
     s->stage = nullptr;
@@ -411,6 +418,13 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
     s->args.kick = c.kick;
     s->args.newton = (c.flags & UTCNB_FLAG_NO_NEWTON) == 0;
     s->args.tile = c.tile > 0 ? c.tile - 1 : tile_default(c.n, cus, s->partial_budget);
+    if (c.tile == 0 && c.precision == UTCNB_PREC_MIXED && !tile_get(s->args.tile).sym) {
+        // a system whose symmetric partial sums exceed the budget in one
+        // launch still takes the symmetric tile, evaluated in chunks
+        const int t = tile_default(c.n, cus, SIZE_MAX);
+        if (tile_get(t).sym)
+            s->args.tile = t;
+    }
 
     switch (c.precision) {
     case UTCNB_PREC_REFERENCE:
@@ -438,11 +452,39 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
             rc = fail(UTCNB_EINVAL, "utcnb_create: symmetric tile needs N padded to 128*K");
         } else {
             const size_t jf = sym_jpart_floats((int)np, t.k), jif = sym_ipart_floats((int)np, t.k, t.s);
-            if ((jf + jif) * sizeof(float) > s->partial_budget)
+            const bool full_fits = (jf + jif) * sizeof(float) <= s->partial_budget;
+            if (c.precision == UTCNB_PREC_MIXED && (!full_fits || (c.flags & UTCNB_FLAG_SYM_CHUNKS))) {
+                // chunks of wave blocks: the fewest (at least 2) whose partial
+                // sums and per-body reaction sums fit the budget
+                const int NW = (int)(np / (64 * (size_t)t.k));
+                int G = 2;
+                size_t cif = 0, cjf = 0;
+                int Rs = 0;
+                for (; G <= NW; ++G) {
+                    const int nw = (NW + G - 1) / G;
+                    Rs = sym_share_splits(nw, t.s, t.k, NW);
+                    cif = (size_t)Rs * np * 3;
+                    cjf = sym_chunk_jpart_floats(nw, NW, t.k);
+                    if ((cif + cjf) * sizeof(float) + 3 * n * sizeof(double) <= s->partial_budget)
+                        break;
+                }
+                if (G > NW) {
+                    rc = fail(UTCNB_EINVAL, "utcnb_create: N too large for a symmetric tile even in chunks");
+                } else {
+                    rc = dalloc(&s->cipart, cif) || dalloc(&s->cjpart, cjf) || dalloc(&s->react, 3 * n);
+                    s->args.chunks = G;
+                    s->args.chunk_r = Rs;
+                    s->args.cipart = s->cipart;
+                    s->args.cjpart = s->cjpart;
+                    s->args.react = s->react;
+                }
+            } else if (!full_fits) {
                 rc = fail(UTCNB_EINVAL, "utcnb_create: N too large for a symmetric tile (partial sums "
-                                        "above a quarter of the device, at most 64 GiB); use a one-pass tile");
-            else
+                                        "above a quarter of the device, at most 64 GiB); use a one-pass tile "
+                                        "(or MIXED, which evaluates the symmetric step in chunks)");
+            } else {
                 rc = dalloc(&s->ipart, jif) || dalloc(&s->jpart, jf);
+            }
         }
         s->args.ipart = s->ipart;
         s->args.jpart = s->jpart;
@@ -620,6 +662,14 @@ int utcnb_sync(utcnb_system* s)
     return UTCNB_OK;
 }
 
+int utcnb_sym_chunks(const utcnb_system* s, int* chunks)
+{
+    if (!s || !chunks)
+        return fail(UTCNB_EINVAL, "utcnb_sym_chunks: null argument");
+    *chunks = s->args.chunks;
+    return UTCNB_OK;
+}
+
 int utcnb_get_tile(const utcnb_system* s, int* tile)
 {
     if (!s || !tile)
@@ -750,6 +800,9 @@ static int share_geom(const utcnb_system* s, int shares, int share, const char* 
     if (s->cfg.precision != UTCNB_PREC_MIXED || !tile_get(s->args.tile).sym)
         return fail(UTCNB_EINVAL, std::string(who) + ": needs a MIXED system with a symmetric tile "
                                                      "(utcnb_step_slice shares any other system)");
+    if (!s->jpart)
+        return fail(UTCNB_EINVAL, std::string(who) + ": the system evaluates its symmetric steps in chunks "
+                                                     "(no whole-system reaction buffer); use utcnb_step_slice");
     const Tile& t = tile_get(s->args.tile);
     const int B = 64 * t.k;
     g->NW = s->n_pad / B;
@@ -798,8 +851,8 @@ int utcnb_share_pairs(utcnb_system* s, int shares, int share, double* react3)
     }
     UTCNB_HIP(hipEventRecord(s->ev0, s->stream));
     hipError_t e = launch_sym_share_pairs(tile_get(s->args.tile), s->args.newton, s->snap[s->cur], s->n, s->n_pad,
-                                          s->args.eps2f, shares, share, g.Rs, s->share_ipart, s->jpart, react3,
-                                          s->stream);
+                                          s->args.eps2f, shares, share, g.Rs, s->share_ipart, s->jpart, false,
+                                          react3, false, s->stream);
     if (e != hipSuccess)
         return fail(UTCNB_EHIP, std::string("utcnb_share_pairs: ") + hipGetErrorString(e));
     UTCNB_HIP(hipEventRecord(s->ev1, s->stream));

@@ -74,6 +74,10 @@ int tile_default(int64_t n, int cu_count, size_t partial_budget)
 template <class Epi>
 hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, int n, int n_pad, float eps2,
                       float* ipart, float* jpart, const Epi& epi, hipStream_t st);
+template <class Epi>
+hipError_t launch_sym_chunked(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
+                              int chunks, int Rs, float* ipart, float* jchunk, double* react3, const Epi& epi,
+                              hipStream_t st);
 
 hipError_t launch_fp32_step(const StepArgs& a, const float4* pos_old, float4* pos_new,
                             float4* vel, hipStream_t st)
@@ -91,6 +95,9 @@ hipError_t launch_mixed_step(const StepArgs& a, const float4* snap_old, float4* 
                              double4* pos, double4* vel, hipStream_t st)
 {
     EpiMixed epi{pos, vel, snap_new, a.dt, a.damping, a.kick};
+    if (tile_get(a.tile).sym && a.chunks > 0)
+        return launch_sym_chunked(tile_get(a.tile), a.newton, snap_old, a.n, a.n_pad, a.eps2f, a.chunks, a.chunk_r,
+                                  a.cipart, a.cjpart, a.react, epi, st);
     if (tile_get(a.tile).sym)
         return launch_sym(tile_get(a.tile), a.newton, true, snap_old, a.n, a.n_pad, a.eps2f, a.ipart,
                           a.jpart, epi, st);
@@ -148,6 +155,12 @@ hipError_t launch_fp32_accel(const StepArgs& a, const float4* src, bool compensa
                              double* acc3, hipStream_t st)
 {
     EpiAccel epi{acc3};
+    if (tile_get(a.tile).sym && a.chunks > 0) {
+        if (!compensated)
+            return hipErrorInvalidValue;  // the chunked step sums its partials in FP64
+        return launch_sym_chunked(tile_get(a.tile), a.newton, src, a.n, a.n_pad, a.eps2f, a.chunks, a.chunk_r,
+                                  a.cipart, a.cjpart, a.react, epi, st);
+    }
     if (tile_get(a.tile).sym)
         return launch_sym(tile_get(a.tile), a.newton, compensated, src, a.n, a.n_pad, a.eps2f, a.ipart,
                           a.jpart, epi, st);

@@ -60,6 +60,12 @@ struct StepArgs {
     int tile;       // tile id
     float* ipart;   // symmetric tiles: target-side partials [R][n_pad][3]
     float* jpart;   // symmetric tiles: reaction partials
+    // symmetric tiles evaluated in chunks of wave blocks (0: one launch)
+    int chunks;
+    int chunk_r;    // waves per wave block in a chunk
+    float* cipart;  // target-side partials [chunk_r][n_pad][3]
+    float* cjpart;  // reaction partials of one chunk's wave blocks
+    double* react;  // reactions summed per body over the chunks, FP64 [3n]
 };
 // Workspace of a symmetric tile (floats).
 size_t sym_ipart_floats(int n_pad, int K, int R);
@@ -127,9 +133,13 @@ void sym_share_blocks(int NW, int shares, int share, int* w0, int* w1);
 int sym_share_splits(int nw_share, int R, int K, int NW);
 // The share's pair work (target-side partials into ipart [Rs][n_pad][3],
 // reactions into jpart) and its reactions summed per body into react3 (FP64 [3n]).
+// jpart_share_rows: jpart holds the share's wave blocks' rows only (else all NW);
+// accumulate: react3 += the share's reactions (else react3 =).
 hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
-                                  int shares, int share, int Rs, float* ipart, float* jpart, double* react3,
-                                  hipStream_t st);
+                                  int shares, int share, int Rs, float* ipart, float* jpart, bool jpart_share_rows,
+                                  double* react3, bool accumulate, hipStream_t st);
+// Reaction partials of a chunk of nw wave blocks (floats).
+inline size_t sym_chunk_jpart_floats(int nw, int NW, int K) { return (size_t)nw * (NW / 2) * 64 * K * 3; }
 // Bodies [start, start + count): target-side partials + react_sum3 [3*count],
 // then the MIXED epilogue (FP64 state in place, new snapshot into snap_new).
 hipError_t launch_sym_share_finish_mixed(const StepArgs& a, int Rs, int start, int count, const float* ipart,

@@ -200,12 +200,13 @@ __device__ __forceinline__ void sym_own_at(int kb, bool uni, float4 s, const flo
 
 // The work of wave r of wave block w of one system: src = the system's
 // padded bodies (NW wave blocks of 64K); ipart: [R][n_pad][3] target-side
-// sums; jpart: [NW][NW/2][64K][3] reactions.  COMP: the target-side block
+// sums; jpart: [NW][NW/2][64K][3] reactions, or the rows of wave blocks
+// jbase, jbase + 1, ... only (a chunk of a step).  COMP: the target-side block
 // partials are Kahan-added (the compensated paths); otherwise plain FP32 sums.
 template <int K, bool NEWTON, bool COMP>
 __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW, int R, int w, int r,
                                          float eps2, float* __restrict__ ipart,
-                                         float* __restrict__ jpart, int lane)
+                                         float* __restrict__ jpart, int lane, int jbase = 0)
 {
     const int half = NW / 2;
     const int n_pad = NW * 64 * K;
@@ -280,7 +281,7 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
             sym_block<K, NEWTON, COMP, true>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
         else
             sym_block<K, NEWTON, COMP, false>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
-        const size_t o = (((size_t)w * half + d) * (64 * K) + kb * 64 + lane) * 3;
+        const size_t o = (((size_t)(w - jbase) * half + d) * (64 * K) + kb * 64 + lane) * 3;
         jpart[o + 0] = gx;
         jpart[o + 1] = gy;
         jpart[o + 2] = gz;
@@ -461,21 +462,25 @@ hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, 
 // integrates them.  On one device, running every share and summing the react3
 // arrays gives the full step's sums in another FP64 order.
 
+// jbase: the first wave block whose reaction rows jpart holds (0 for a
+// whole-system buffer, w0 for a buffer of the share's rows only).
 template <int K, bool NEWTON>
 __global__ __launch_bounds__(256) void sym_share_kernel(const float4* __restrict__ src, int NW, int R, int w0,
                                                         int nwaves, float eps2, float* __restrict__ ipart,
-                                                        float* __restrict__ jpart)
+                                                        float* __restrict__ jpart, int jbase)
 {
     const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (gw >= nwaves)
         return;  // whole waves only
-    sym_wave<K, NEWTON, true>(src, NW, R, w0 + gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
+    sym_wave<K, NEWTON, true>(src, NW, R, w0 + gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63, jbase);
 }
 
 // react3[b] (FP64, bodies b < n) = the reactions on body b from the wave
 // blocks [w0, w1), in increasing block order; 0 where none of them reaches b.
+// With accumulate, the previous react3[b] plus that sum.
 __global__ __launch_bounds__(256) void sym_react_kernel(const float* __restrict__ jpart, int n, int NW, int K,
-                                                        int w0, int w1, double* __restrict__ react3)
+                                                        int w0, int w1, int jbase, bool accumulate,
+                                                        double* __restrict__ react3)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n)
@@ -489,14 +494,21 @@ __global__ __launch_bounds__(256) void sym_react_kernel(const float* __restrict_
             d += NW;
         if (d >= half || (2 * (d + 1) == NW && w >= half))
             continue;  // v is not in w's half ring (or w's upper half skipped it)
-        const float* p = jpart + (((size_t)w * half + d) * B + loc) * 3;
+        const float* p = jpart + (((size_t)(w - jbase) * half + d) * B + loc) * 3;
         s0 += p[0];
         s1 += p[1];
         s2 += p[2];
     }
-    react3[3 * (size_t)b + 0] = s0;
-    react3[3 * (size_t)b + 1] = s1;
-    react3[3 * (size_t)b + 2] = s2;
+    double* o = react3 + 3 * (size_t)b;
+    if (accumulate) {
+        o[0] += s0;
+        o[1] += s1;
+        o[2] += s2;
+    } else {
+        o[0] = s0;
+        o[1] = s1;
+        o[2] = s2;
+    }
 }
 
 // Bodies [start, start + count): their R target-side partials summed in FP64
@@ -550,8 +562,8 @@ int sym_share_splits(int nw_share, int R, int K, int NW)
 }
 
 hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
-                                  int shares, int share, int Rs, float* ipart, float* jpart, double* react3,
-                                  hipStream_t st)
+                                  int shares, int share, int Rs, float* ipart, float* jpart, bool jpart_share_rows,
+                                  double* react3, bool accumulate, hipStream_t st)
 {
     const int K = t.k;
     if (n_pad % (128 * K) != 0 || !ipart || !jpart)
@@ -560,15 +572,16 @@ hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src,
     int w0, w1;
     sym_share_blocks(NW, shares, share, &w0, &w1);
     const int nwaves = (w1 - w0) * Rs;
+    const int jbase = jpart_share_rows ? w0 : 0;
     if (nwaves > 0) {
         const dim3 grid((nwaves + 3) / 4), block(256);
 #define SHARE_LAUNCH(KK)                                                                                  \
     if (newton)                                                                                           \
         hipLaunchKernelGGL((sym_share_kernel<KK, true>), grid, block, 0, st, src, NW, Rs, w0, nwaves, eps2, \
-                           ipart, jpart);                                                                 \
+                           ipart, jpart, jbase);                                                          \
     else                                                                                                  \
         hipLaunchKernelGGL((sym_share_kernel<KK, false>), grid, block, 0, st, src, NW, Rs, w0, nwaves, eps2, \
-                           ipart, jpart);
+                           ipart, jpart, jbase);
         if (K == 2) {
             SHARE_LAUNCH(2)
         } else if (K == 4) {
@@ -583,25 +596,59 @@ hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src,
         if (e != hipSuccess)
             return e;
     }
-    hipLaunchKernelGGL(sym_react_kernel, dim3((n + 255) / 256), dim3(256), 0, st, jpart, n, NW, K, w0, w1, react3);
+    hipLaunchKernelGGL(sym_react_kernel, dim3((n + 255) / 256), dim3(256), 0, st, jpart, n, NW, K, w0, w1, jbase,
+                       accumulate, react3);
     return hipGetLastError();
 }
+
+template <class Epi>
+static hipError_t launch_sym_finish(int n_pad, int Rs, int start, int count, const float* ipart,
+                                    const double* react_sum3, const Epi& epi, hipStream_t st)
+{
+    const int T = sym_reduce_lanes(count);
+    const dim3 grid((unsigned)(((long long)count * T + 255) / 256)), block(256);
+    switch (T) {
+    case 1: hipLaunchKernelGGL((sym_share_finish_kernel<1, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
+    case 2: hipLaunchKernelGGL((sym_share_finish_kernel<2, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
+    case 4: hipLaunchKernelGGL((sym_share_finish_kernel<4, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
+    case 8: hipLaunchKernelGGL((sym_share_finish_kernel<8, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
+    default: hipLaunchKernelGGL((sym_share_finish_kernel<16, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
+    }
+    return hipGetLastError();
+}
+
+// A whole step of the symmetric tile evaluated in `chunks` chunks of wave
+// blocks, one after the other: each chunk's pair work (target-side partials
+// into ipart, R_share rows; its reactions into jchunk, a buffer of one chunk's
+// rows) and the reactions summed per body into react3, accumulated over the
+// chunks in order; then every body's target-side partials plus react3 and
+// the epilogue.  The same pair work as one launch of the symmetric kernel
+// with partial-sum memory for one chunk's reactions.
+template <class Epi>
+hipError_t launch_sym_chunked(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
+                              int chunks, int Rs, float* ipart, float* jchunk, double* react3, const Epi& epi,
+                              hipStream_t st)
+{
+    for (int g = 0; g < chunks; ++g) {
+        hipError_t e = launch_sym_share_pairs(t, newton, src, n, n_pad, eps2, chunks, g, Rs, ipart, jchunk, true,
+                                              react3, g > 0, st);
+        if (e != hipSuccess)
+            return e;
+    }
+    return launch_sym_finish(n_pad, Rs, 0, n, ipart, react3, epi, st);
+}
+
+template hipError_t launch_sym_chunked<EpiMixed>(const Tile&, bool, const float4*, int, int, float, int, int, float*,
+                                                 float*, double*, const EpiMixed&, hipStream_t);
+template hipError_t launch_sym_chunked<EpiAccel>(const Tile&, bool, const float4*, int, int, float, int, int, float*,
+                                                 float*, double*, const EpiAccel&, hipStream_t);
 
 hipError_t launch_sym_share_finish_mixed(const StepArgs& a, int Rs, int start, int count, const float* ipart,
                                          const double* react_sum3, float4* snap_new, double4* pos, double4* vel,
                                          hipStream_t st)
 {
     EpiMixed epi{pos, vel, snap_new, a.dt, a.damping, a.kick};
-    const int T = sym_reduce_lanes(count);
-    const dim3 grid((unsigned)(((long long)count * T + 255) / 256)), block(256);
-    switch (T) {
-    case 1: hipLaunchKernelGGL((sym_share_finish_kernel<1, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
-    case 2: hipLaunchKernelGGL((sym_share_finish_kernel<2, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
-    case 4: hipLaunchKernelGGL((sym_share_finish_kernel<4, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
-    case 8: hipLaunchKernelGGL((sym_share_finish_kernel<8, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
-    default: hipLaunchKernelGGL((sym_share_finish_kernel<16, EpiMixed>), grid, block, 0, st, ipart, a.n_pad, Rs, start, count, react_sum3, epi); break;
-    }
-    return hipGetLastError();
+    return launch_sym_finish(a.n_pad, Rs, start, count, ipart, react_sum3, epi, st);
 }
 
 hipError_t launch_batch_counter_reset(int* counter, hipStream_t st)
