@@ -97,6 +97,30 @@ def test_kernel_level_slice_bit_exact_f64():
         assert np.array_equal(d_vel.cpu().numpy(), ov)
 
 
+def test_reference_accel_slice_bit_exact():
+    """utcnb_accel_slice of a REFERENCE handle (FP32 and FP64 state): the
+    reference's field rows, bit for bit the oracle's."""
+    n = 5003
+    c = plummer(n, eps=0.01, dt=1e-3)
+    start, cnt = U.partition(n, 4)[2]
+    say(f"case: reference accel_slice n={n} start={start} count={cnt}")
+    pos32 = c["pos"].astype(np.float32)
+    eps2f = np.float32(0.01) * np.float32(0.01)
+    with U.System(n, U.PREC_REFERENCE, float(eps2f), 1e-3, kick=U.KICK_REFERENCE, state_dtype=U.F32) as s:
+        s.set_state(pos32)
+        got = s.accel_slice(start, cnt)
+        ptr, dt = s.position_buffer(0)
+        assert ptr and dt == U.F32
+    want = O.gravitation_rows_f32(pos32, eps2f, start, start + cnt, nthreads=8)[start:start + cnt]
+    assert np.array_equal(got, want.astype(np.float64))
+    with U.System(n, U.PREC_REFERENCE, 1e-4, 1e-3, kick=U.KICK_REFERENCE, state_dtype=U.F64) as s:
+        s.set_state(c["pos"])
+        got = s.accel_slice(start, cnt)
+        assert s.position_buffer(1)[1] == U.F64
+    want = O.gravitation_f64(c["pos"], 1e-4)[start:start + cnt]
+    assert np.array_equal(got, want)
+
+
 def test_kernel_level_slice_rejects_bad_ranges():
     L = U.lib()
     for start, cnt in ((-1, 4), (0, 0), (8, 3), (5, 6)):
