@@ -311,7 +311,7 @@ def slice_per_gpu(U, c, n, K, shares=8):
             continue
         with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], flags=U.FLAG_NO_NEWTON, tile=tid) as s:
             s.set_state(c["pos"], c["vel"])
-            for _ in range(2):  # warm-up
+            for _ in range(10):  # warm-up
                 s.step_slice(start, cnt)
             s.sync()
             t0 = time.perf_counter()
@@ -356,7 +356,9 @@ def slice_per_gpu(U, c, n, K, shares=8):
             s0, c0 = s.share_range(shares, 0)
             rsum = total[s0:s0 + c0].contiguous()
             torch.cuda.synchronize()
-            for _ in range(2):  # warm-up
+            # warm-up: about 50 ms of this loop (its first ~20 ms run ~10% slower
+            # on MI355X, as the headline leg's first steps do before its warm-up)
+            for _ in range(30):
                 s.share_pairs(shares, 0, reacts[0].data_ptr())
                 s.share_finish(shares, 0, rsum.data_ptr())
             s.sync()

@@ -851,8 +851,8 @@ int utcnb_share_pairs(utcnb_system* s, int shares, int share, double* react3)
     }
     UTCNB_HIP(hipEventRecord(s->ev0, s->stream));
     hipError_t e = launch_sym_share_pairs(tile_get(s->args.tile), s->args.newton, s->snap[s->cur], s->n, s->n_pad,
-                                          s->args.eps2f, shares, share, g.Rs, s->share_ipart, s->jpart, false,
-                                          react3, false, s->stream);
+                                          s->args.eps2f, shares, share, g.Rs, s->share_ipart, true, s->jpart,
+                                          false, react3, false, s->stream);
     if (e != hipSuccess)
         return fail(UTCNB_EHIP, std::string("utcnb_share_pairs: ") + hipGetErrorString(e));
     UTCNB_HIP(hipEventRecord(s->ev1, s->stream));
@@ -875,8 +875,8 @@ int utcnb_share_finish(utcnb_system* s, int shares, int share, const double* rea
     if (g.count <= 0)
         return UTCNB_OK;
     UTCNB_HIP(hipEventRecord(s->ev0, s->stream));
-    hipError_t e = launch_sym_share_finish_mixed(s->args, g.Rs, g.start, g.count, s->share_ipart, react_sum3,
-                                                 s->snap[1 - s->cur], s->p64[0], s->v64, s->stream);
+    hipError_t e = launch_sym_share_finish_mixed(s->args, shares, share, g.Rs, g.start, g.count, s->share_ipart,
+                                                 react_sum3, s->snap[1 - s->cur], s->p64[0], s->v64, s->stream);
     if (e != hipSuccess)
         return fail(UTCNB_EHIP, std::string("utcnb_share_finish: ") + hipGetErrorString(e));
     UTCNB_HIP(hipEventRecord(s->ev1, s->stream));

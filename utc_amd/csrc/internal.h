@@ -135,16 +135,18 @@ int sym_share_splits(int nw_share, int R, int K, int NW);
 // reactions into jpart) and its reactions summed per body into react3 (FP64 [3n]).
 // jpart_share_rows: jpart holds the share's wave blocks' rows only (else all NW);
 // accumulate: react3 += the share's reactions (else react3 =).
+// compact_ipart: the share's target-side partials go to its own contiguous
+// region of ipart (else to the whole system's row layout).
 hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
-                                  int shares, int share, int Rs, float* ipart, float* jpart, bool jpart_share_rows,
-                                  double* react3, bool accumulate, hipStream_t st);
+                                  int shares, int share, int Rs, float* ipart, bool compact_ipart, float* jpart,
+                                  bool jpart_share_rows, double* react3, bool accumulate, hipStream_t st);
 // Reaction partials of a chunk of nw wave blocks (floats).
 inline size_t sym_chunk_jpart_floats(int nw, int NW, int K) { return (size_t)nw * (NW / 2) * 64 * K * 3; }
 // Bodies [start, start + count): target-side partials + react_sum3 [3*count],
 // then the MIXED epilogue (FP64 state in place, new snapshot into snap_new).
-hipError_t launch_sym_share_finish_mixed(const StepArgs& a, int Rs, int start, int count, const float* ipart,
-                                         const double* react_sum3, float4* snap_new, double4* pos, double4* vel,
-                                         hipStream_t st);
+hipError_t launch_sym_share_finish_mixed(const StepArgs& a, int shares, int share, int Rs, int start, int count,
+                                         const float* ipart, const double* react_sum3, float4* snap_new,
+                                         double4* pos, double4* vel, hipStream_t st);
 
 // ---- adjoint (adjoint.hip) ------------------------------------------------
 // jrec: 2 records per body {x,y,z,m}{ax,ay,az,0}, n_pad bodies.

@@ -201,12 +201,15 @@ __device__ __forceinline__ void sym_own_at(int kb, bool uni, float4 s, const flo
 // The work of wave r of wave block w of one system: src = the system's
 // padded bodies (NW wave blocks of 64K); ipart: [R][n_pad][3] target-side
 // sums; jpart: [NW][NW/2][64K][3] reactions, or the rows of wave blocks
-// jbase, jbase + 1, ... only (a chunk of a step).  COMP: the target-side block
+// jbase, jbase + 1, ... only (a chunk of a step).  ipart may also hold a
+// range of bodies only: body b of row r is record r * istride + b - ibody0
+// (istride < 0: n_pad, the whole system).  COMP: the target-side block
 // partials are Kahan-added (the compensated paths); otherwise plain FP32 sums.
 template <int K, bool NEWTON, bool COMP>
 __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW, int R, int w, int r,
                                          float eps2, float* __restrict__ ipart,
-                                         float* __restrict__ jpart, int lane, int jbase = 0)
+                                         float* __restrict__ jpart, int lane, int jbase = 0,
+                                         long long istride = -1, int ibody0 = 0)
 {
     const int half = NW / 2;
     const int n_pad = NW * 64 * K;
@@ -288,7 +291,8 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const size_t o = ((size_t)r * n_pad + w * 64 * K + k * 64 + lane) * 3;
+        const size_t o = ((size_t)r * (size_t)(istride < 0 ? n_pad : istride) + (w * 64 * K - ibody0) + k * 64 +
+                          lane) * 3;
         ipart[o + 0] = ax[k] - cx[k];
         ipart[o + 1] = ay[k] - cy[k];
         ipart[o + 2] = az[k] - cz[k];
@@ -464,15 +468,18 @@ hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, 
 
 // jbase: the first wave block whose reaction rows jpart holds (0 for a
 // whole-system buffer, w0 for a buffer of the share's rows only).
+// ipart: rows of istride bodies starting at body ibody0 (see sym_wave).
 template <int K, bool NEWTON>
 __global__ __launch_bounds__(256) void sym_share_kernel(const float4* __restrict__ src, int NW, int R, int w0,
                                                         int nwaves, float eps2, float* __restrict__ ipart,
-                                                        float* __restrict__ jpart, int jbase)
+                                                        float* __restrict__ jpart, int jbase, long long istride,
+                                                        int ibody0)
 {
     const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (gw >= nwaves)
         return;  // whole waves only
-    sym_wave<K, NEWTON, true>(src, NW, R, w0 + gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63, jbase);
+    sym_wave<K, NEWTON, true>(src, NW, R, w0 + gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63, jbase,
+                              istride, ibody0);
 }
 
 // react3[b] (FP64, bodies b < n) = the reactions on body b from the wave
@@ -511,13 +518,14 @@ __global__ __launch_bounds__(256) void sym_react_kernel(const float* __restrict_
     }
 }
 
-// Bodies [start, start + count): their R target-side partials summed in FP64
-// by T lanes per body (lane t takes rows t, t + T, ...; a fixed butterfly
-// combines the lanes, as in sym_reduce.h), plus react_sum3[j] (the reactions
-// summed over the shares), then the epilogue.
+// Bodies [start, start + count): their R target-side partials (rows of
+// istride bodies from body ibody0) summed in FP64 by T lanes per body (lane t
+// takes rows t, t + T, ...; a fixed butterfly combines the lanes, as in
+// sym_reduce.h), plus react_sum3[j] (the reactions summed over the shares),
+// then the epilogue.
 template <int T, class Epi>
-__global__ __launch_bounds__(256) void sym_share_finish_kernel(const float* __restrict__ ipart, int n_pad, int R,
-                                                               int start, int count,
+__global__ __launch_bounds__(256) void sym_share_finish_kernel(const float* __restrict__ ipart, long long istride,
+                                                               int ibody0, int R, int start, int count,
                                                                const double* __restrict__ react_sum3, Epi epi)
 {
     int j, t;
@@ -526,7 +534,7 @@ __global__ __launch_bounds__(256) void sym_share_finish_kernel(const float* __re
     double s[3] = {0.0, 0.0, 0.0};
 #pragma unroll 4
     for (int r = t; r < R; r += T)
-        part_add(s, ((const Part3*)ipart)[(size_t)r * n_pad + b]);
+        part_add(s, ((const Part3*)ipart)[(size_t)r * (size_t)istride + (b - ibody0)]);
 #pragma unroll
     for (int off = 32; off >= 64 / T; off >>= 1)
 #pragma unroll
@@ -561,9 +569,23 @@ int sym_share_splits(int nw_share, int R, int K, int NW)
     return (int)((Q + u - 1) / u);
 }
 
+// The target-side partials of share `share` in a buffer of Rs x n_pad
+// records: the whole system's rows (compact = false: body b of row r at
+// r * n_pad + b) or the share's own region (compact: Rs rows of the share's
+// bodies, contiguous, at record Rs * w0 * 64K; the shares' regions tile the
+// buffer).
+static void share_ipart_layout(int n_pad, int K, int Rs, int w0, int w1, bool compact, size_t* base,
+                               long long* istride, int* ibody0)
+{
+    const int B = 64 * K;
+    *base = compact ? (size_t)Rs * w0 * B * 3 : 0;
+    *istride = compact ? (long long)(w1 - w0) * B : n_pad;
+    *ibody0 = compact ? w0 * B : 0;
+}
+
 hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src, int n, int n_pad, float eps2,
-                                  int shares, int share, int Rs, float* ipart, float* jpart, bool jpart_share_rows,
-                                  double* react3, bool accumulate, hipStream_t st)
+                                  int shares, int share, int Rs, float* ipart, bool compact_ipart, float* jpart,
+                                  bool jpart_share_rows, double* react3, bool accumulate, hipStream_t st)
 {
     const int K = t.k;
     if (n_pad % (128 * K) != 0 || !ipart || !jpart)
@@ -573,15 +595,19 @@ hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src,
     sym_share_blocks(NW, shares, share, &w0, &w1);
     const int nwaves = (w1 - w0) * Rs;
     const int jbase = jpart_share_rows ? w0 : 0;
+    size_t ibase;
+    long long istride;
+    int ibody0;
+    share_ipart_layout(n_pad, K, Rs, w0, w1, compact_ipart, &ibase, &istride, &ibody0);
     if (nwaves > 0) {
         const dim3 grid((nwaves + 3) / 4), block(256);
 #define SHARE_LAUNCH(KK)                                                                                  \
     if (newton)                                                                                           \
         hipLaunchKernelGGL((sym_share_kernel<KK, true>), grid, block, 0, st, src, NW, Rs, w0, nwaves, eps2, \
-                           ipart, jpart, jbase);                                                          \
+                           ipart + ibase, jpart, jbase, istride, ibody0);                                 \
     else                                                                                                  \
         hipLaunchKernelGGL((sym_share_kernel<KK, false>), grid, block, 0, st, src, NW, Rs, w0, nwaves, eps2, \
-                           ipart, jpart, jbase);
+                           ipart + ibase, jpart, jbase, istride, ibody0);
         if (K == 2) {
             SHARE_LAUNCH(2)
         } else if (K == 4) {
@@ -602,18 +628,21 @@ hipError_t launch_sym_share_pairs(const Tile& t, bool newton, const float4* src,
 }
 
 template <class Epi>
-static hipError_t launch_sym_finish(int n_pad, int Rs, int start, int count, const float* ipart,
+static hipError_t launch_sym_finish(const float* ipart, long long istride, int ibody0, int Rs, int start, int count,
                                     const double* react_sum3, const Epi& epi, hipStream_t st)
 {
     const int T = sym_reduce_lanes(count);
     const dim3 grid((unsigned)(((long long)count * T + 255) / 256)), block(256);
+#define FINISH(TT) hipLaunchKernelGGL((sym_share_finish_kernel<TT, Epi>), grid, block, 0, st, ipart, istride, ibody0, \
+                                      Rs, start, count, react_sum3, epi)
     switch (T) {
-    case 1: hipLaunchKernelGGL((sym_share_finish_kernel<1, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
-    case 2: hipLaunchKernelGGL((sym_share_finish_kernel<2, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
-    case 4: hipLaunchKernelGGL((sym_share_finish_kernel<4, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
-    case 8: hipLaunchKernelGGL((sym_share_finish_kernel<8, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
-    default: hipLaunchKernelGGL((sym_share_finish_kernel<16, Epi>), grid, block, 0, st, ipart, n_pad, Rs, start, count, react_sum3, epi); break;
+    case 1: FINISH(1); break;
+    case 2: FINISH(2); break;
+    case 4: FINISH(4); break;
+    case 8: FINISH(8); break;
+    default: FINISH(16); break;
     }
+#undef FINISH
     return hipGetLastError();
 }
 
@@ -630,12 +659,12 @@ hipError_t launch_sym_chunked(const Tile& t, bool newton, const float4* src, int
                               hipStream_t st)
 {
     for (int g = 0; g < chunks; ++g) {
-        hipError_t e = launch_sym_share_pairs(t, newton, src, n, n_pad, eps2, chunks, g, Rs, ipart, jchunk, true,
-                                              react3, g > 0, st);
+        hipError_t e = launch_sym_share_pairs(t, newton, src, n, n_pad, eps2, chunks, g, Rs, ipart, false, jchunk,
+                                              true, react3, g > 0, st);
         if (e != hipSuccess)
             return e;
     }
-    return launch_sym_finish(n_pad, Rs, 0, n, ipart, react3, epi, st);
+    return launch_sym_finish(ipart, n_pad, 0, Rs, 0, n, react3, epi, st);
 }
 
 template hipError_t launch_sym_chunked<EpiMixed>(const Tile&, bool, const float4*, int, int, float, int, int, float*,
@@ -643,12 +672,20 @@ template hipError_t launch_sym_chunked<EpiMixed>(const Tile&, bool, const float4
 template hipError_t launch_sym_chunked<EpiAccel>(const Tile&, bool, const float4*, int, int, float, int, int, float*,
                                                  float*, double*, const EpiAccel&, hipStream_t);
 
-hipError_t launch_sym_share_finish_mixed(const StepArgs& a, int Rs, int start, int count, const float* ipart,
-                                         const double* react_sum3, float4* snap_new, double4* pos, double4* vel,
-                                         hipStream_t st)
+hipError_t launch_sym_share_finish_mixed(const StepArgs& a, int shares, int share, int Rs, int start, int count,
+                                         const float* ipart, const double* react_sum3, float4* snap_new,
+                                         double4* pos, double4* vel, hipStream_t st)
 {
+    const Tile& t = tile_get(a.tile);
+    const int NW = a.n_pad / (64 * t.k);
+    int w0, w1;
+    sym_share_blocks(NW, shares, share, &w0, &w1);
+    size_t ibase;
+    long long istride;
+    int ibody0;
+    share_ipart_layout(a.n_pad, t.k, Rs, w0, w1, true, &ibase, &istride, &ibody0);
     EpiMixed epi{pos, vel, snap_new, a.dt, a.damping, a.kick};
-    return launch_sym_finish(a.n_pad, Rs, start, count, ipart, react_sum3, epi, st);
+    return launch_sym_finish(ipart + ibase, istride, ibody0, Rs, start, count, react_sum3, epi, st);
 }
 
 hipError_t launch_batch_counter_reset(int* counter, hipStream_t st)
