@@ -1,4 +1,5 @@
-"""GPU tests of the C++ drivers over the UTC task shim.
+"""GPU tests of the C++ drivers over the UTC task shim (nbody_main,
+nbody_mtask, nbody_mgpu).
 
 The reference's golden frame file (benchapps/nbody/seq/nbody_outpos.txt) is the
 output of `-n 1024 -l 100 -O file` with FP32 arithmetic; nbody_main must
@@ -140,6 +141,32 @@ def test_nbody_mtask_runs_concurrent_tasks(tmp_path):
     assert "Number of tasks: 4" in p.stdout
     rec = (tmp_path / "time_record.txt").read_text().splitlines()
     assert len(rec[0]) == 80  # five fields
+
+
+@pytest.mark.parametrize("threads,memtype", [(4, 0), (3, 1), (8, 2)])
+def test_nbody_mgpu_reproduces_golden_file(tmp_path, threads, memtype):
+    """The multi-GPU driver (BodySystemMGPU: each task thread advances its slice
+    of the bodies against all of them and the slices meet in host buffers every
+    step; here the threads share the one GPU) in FP32: byte for byte the
+    reference's golden frame file, like the single-task driver."""
+    run([str(BIN / "nbody_mgpu"), "-f", "-t", str(threads), "-m", str(memtype), "-n", "1024", "-l", "100",
+         "-O", "out.txt", "-v"], tmp_path)
+    data = (tmp_path / "out.txt").read_bytes()
+    assert hashlib.sha256(data).hexdigest() == SEQ_SHA
+    rec = (tmp_path / "time_record.txt").read_text().splitlines()
+    assert len(rec[0]) == 64  # four 16-wide fields
+
+
+def test_nbody_mgpu_presets_fp64(tmp_path):
+    """FP64 (the reference driver's FTYPE), uneven slices (N = 4096 over 3 and
+    over 7 threads): byte for byte the reference's sequential driver's frame
+    files (tests/golden/preset_frames_fp64.json)."""
+    for threads, preset in ((3, 2), (7, 5)):
+        rec = next(r for r in PRESETS["frames"] if r["n"] == 4096 and r["preset"] == preset)
+        out = tmp_path / f"m{threads}.txt"
+        run([str(BIN / "nbody_mgpu"), "-t", str(threads), "-n", "4096", "-i", str(preset), "-l", str(rec["steps"]),
+             "-O", str(out)], tmp_path)
+        assert hashlib.sha256(out.read_bytes()).hexdigest() == rec["sha256"], (threads, preset)
 
 
 def test_nbody_main_rejects_bad_options(tmp_path):

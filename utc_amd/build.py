@@ -4,6 +4,7 @@ Compiles, in-tree:
   utc_amd/lib/libutcnb.so      HIP kernels + the C ABI of include/utcnb.h
   utc_amd/lib/nbody_main       the UTC nbody driver over the C++ task shim
   utc_amd/lib/nbody_mtask      the multi-task driver
+  utc_amd/lib/nbody_mgpu       the multi-GPU driver (one task thread per GPU)
   utc_amd/lib/test_utc_shim    CPU self-test of the task shim
 and the CPU oracle (oracle/Makefile).  Every HIP translation unit is built
 with the packed-FP32 instructions disabled
@@ -112,6 +113,7 @@ def build_host_programs() -> list[Path]:
     targets = {
         "nbody_main": [APPS / "nbody_main.cpp", APPS / "body_task_sgpu.cpp", APPS / "task.cpp"],
         "nbody_mtask": [APPS / "nbody_mtask.cpp", APPS / "body_task_sgpu.cpp", APPS / "task.cpp"],
+        "nbody_mgpu": [APPS / "nbody_mgpu.cpp", APPS / "body_task_mgpu.cpp", APPS / "task.cpp"],
         "test_utc_shim": [UTC / "test" / "test_utc_shim.cpp"],
     }
 
@@ -125,7 +127,7 @@ def build_host_programs() -> list[Path]:
               *map(str, shim_src), *link])
         return exe
 
-    with cf.ThreadPoolExecutor(max_workers=3) as ex:
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
         for r in ex.map(one, targets.items()):
             if r is not None:
                 outs.append(r)
