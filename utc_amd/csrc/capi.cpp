@@ -451,7 +451,8 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
         if (np % (size_t)(128 * t.k) != 0) {
             rc = fail(UTCNB_EINVAL, "utcnb_create: symmetric tile needs N padded to 128*K");
         } else {
-            const size_t jf = sym_jpart_floats((int)np, t.k), jif = sym_ipart_floats((int)np, t.k, t.s);
+            const size_t jf = sym_jpart_floats((int)np, t.k),
+                         jif = sym_ipart_floats((int)np, t.k, sym_tile_r(t, (int)np));
             const bool full_fits = (jf + jif) * sizeof(float) <= s->partial_budget;
             if (c.precision == UTCNB_PREC_MIXED && (!full_fits || (c.flags & UTCNB_FLAG_SYM_CHUNKS))) {
                 // chunks of wave blocks: the fewest (at least 2) whose partial

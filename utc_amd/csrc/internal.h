@@ -67,6 +67,8 @@ struct StepArgs {
     float* cjpart;  // reaction partials of one chunk's wave blocks
     double* react;  // reactions summed per body over the chunks, FP64 [3n]
 };
+// Waves per wave block of a symmetric tile for a system of n_pad bodies.
+int sym_tile_r(const Tile& t, int n_pad);
 // Workspace of a symmetric tile (floats).
 size_t sym_ipart_floats(int n_pad, int K, int R);
 size_t sym_jpart_floats(int n_pad, int K);

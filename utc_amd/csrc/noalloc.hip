@@ -116,7 +116,7 @@ extern "C" int utcnb_accel_noalloc(const float* pos4, int n, double eps2, int pr
         size_t need = src_bytes;
         if (tl.sym) {
             ipart_off = need;
-            need += round256(sym_ipart_floats((int)n_pad, tl.k, tl.s) * sizeof(float));
+            need += round256(sym_ipart_floats((int)n_pad, tl.k, sym_tile_r(tl, (int)n_pad)) * sizeof(float));
             jpart_off = need;
             need += round256(sym_jpart_floats((int)n_pad, tl.k) * sizeof(float));
         }

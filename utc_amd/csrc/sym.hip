@@ -388,6 +388,12 @@ __global__ __launch_bounds__(256) void batch_sym_reduce_kernel(BatchSymGeom g, i
         sym_epilogue(epi, b, sum);  // padding bodies keep their far-away zero-mass records
 }
 
+int sym_tile_r(const Tile& t, int n_pad)
+{
+    (void)n_pad;
+    return t.s;
+}
+
 size_t sym_ipart_floats(int n_pad, int K, int R) { (void)K; return (size_t)R * n_pad * 3; }
 size_t sym_jpart_floats(int n_pad, int K)
 {
@@ -426,7 +432,7 @@ template <class Epi>
 hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, int n, int n_pad, float eps2,
                       float* ipart, float* jpart, const Epi& epi, hipStream_t st)
 {
-    const int K = t.k, R = t.s;
+    const int K = t.k, R = sym_tile_r(t, n_pad);
     if (n_pad % (128 * K) != 0 || !ipart || !jpart)
         return hipErrorInvalidValue;
     const int NW = n_pad / (64 * K);
