@@ -196,7 +196,8 @@ int utcnb_position_buffer(utcnb_system* sys, int which, void** pos4, int* dtype)
  *      on one device, a plain sum);
  *   3. utcnb_share_finish: the share's bodies [start, start + count)
  *      (utcnb_share_range) add their target-side partials to react_sum3
- *      (device FP64 [3*count]) and are integrated; their new positions go to
+ *      (device FP64 [3*count]; may be NULL when count is 0, a share of
+ *      padding bodies only) and are integrated; their new positions go to
  *      the next position buffer;
  *   4. the caller gathers the other shares' positions into that buffer and
  *      calls utcnb_commit_step.

@@ -441,3 +441,64 @@ def test_large_n_takes_chunked_symmetric_steps():
     v_want = c["vel"][:, :3] + a * c["dt"]
     assert relnorm(v1[:, :3], v_want) <= 1e-12
     assert np.all(np.isfinite(p1))
+
+
+# ---- edge cases --------------------------------------------------------------------
+def test_slice_edges_single_bodies():
+    """Slices of one body (the first, the last) and a one-body system, in the
+    reference arithmetic and MIXED: each equals the full step's body."""
+    n = 777
+    c = plummer(n, eps=0.01, dt=1e-3)
+    say("case: one-body slices")
+    for prec in (U.PREC_REFERENCE, U.PREC_MIXED):
+        kick = U.KICK_REFERENCE if prec == U.PREC_REFERENCE else U.KICK_PHYSICAL
+        tid = 0 if prec == U.PREC_REFERENCE else U.tiles().index("k1s16") + 1
+        with U.System(n, prec, 1e-4, 1e-3, kick=kick, tile=tid) as full:
+            full.set_state(c["pos"], c["vel"])
+            full.step(1)
+            pf, vf = full.get_state()
+        with U.System(n, prec, 1e-4, 1e-3, kick=kick, tile=tid) as s:
+            s.set_state(c["pos"], c["vel"])
+            s.step_slice(0, 1)
+            s.step_slice(n - 1, 1)
+            s.commit_step()
+            p, v = s.get_state()
+        for i in (0, n - 1):
+            assert np.array_equal(p[i], pf[i]) and np.array_equal(v[i], vf[i]), (prec, i)
+    with U.System(1, U.PREC_MIXED, 1e-4, 1e-3) as s:
+        s.set_state(c["pos"][:1], c["vel"][:1])
+        s.step_slice(0, 1)
+        s.commit_step()
+        p, _ = s.get_state()
+        a = s.accel_slice(0, 1)
+    assert np.array_equal(a, np.zeros((1, 3)))  # a lone body feels no force
+    assert np.allclose(p[0, :3], c["pos"][0, :3] + c["vel"][0, :3] * 1e-3, rtol=0, atol=1e-15)
+
+
+def test_shares_with_padding_only_shares():
+    """N = 1025 on sym2r32 padded to 2048: 16 wave blocks of 128 bodies, 7 of
+    them padding only; 16 shares, the last 7 with no bodies: the all-shares
+    flow still follows the single-launch step (1e-12), and the chunked step
+    (two chunks) as well."""
+    n, G = 1025, 16
+    c = plummer(n, eps=0.01, dt=1e-3)
+    tid = U.tiles().index("sym2r32") + 1
+    say(f"case: shares with padding-only shares n={n} G={G}")
+    with U.System(n, U.PREC_MIXED, 1e-4, 1e-3, tile=tid) as full:
+        full.set_state(c["pos"], c["vel"])
+        full.step(2)
+        pf, _ = full.get_state()
+    with U.System(n, U.PREC_MIXED, 1e-4, 1e-3, tile=tid) as s:
+        ranges = [s.share_range(G, g) for g in range(G)]
+        assert [k for _, k in ranges[9:]] == [0] * 7 and ranges[8] == (1024, 1)
+        s.set_state(c["pos"], c["vel"])
+        for _ in range(2):
+            _share_step_all(s, G, n)
+        ps, _ = s.get_state()
+    assert relnorm(ps[:, :3], pf[:, :3]) <= 1e-12
+    with U.System(n, U.PREC_MIXED, 1e-4, 1e-3, tile=tid, flags=U.FLAG_SYM_CHUNKS) as s:
+        assert s.sym_chunks() == 2
+        s.set_state(c["pos"], c["vel"])
+        s.step(2)
+        pc, _ = s.get_state()
+    assert relnorm(pc[:, :3], pf[:, :3]) <= 1e-12

@@ -57,6 +57,15 @@ def test_noalloc_field_equals_handle_field(prec, tile_name):
         assert np.array_equal(got, ref), (n, tname, float(np.abs(got - ref).max()))
 
 
+def test_noalloc_single_body():
+    """A lone body: its field is zero, as utcnb_accel gives."""
+    pos32 = np.array([[0.3, -0.2, 0.1, 1.0]], np.float32)
+    say("case: noalloc n=1")
+    ref, (tid, _) = handle_field(pos32, 1e-4, U.PREC_MIXED, 0, 0)
+    got = noalloc_field(pos32, 1e-4, U.PREC_MIXED, tid, 0)
+    assert np.array_equal(got, ref) and not got.any()
+
+
 def test_noalloc_rejects_sizes_beyond_its_scratch():
     import torch
     say("case: noalloc size limit")

@@ -864,17 +864,19 @@ int utcnb_share_pairs(utcnb_system* s, int shares, int share, double* react3)
 
 int utcnb_share_finish(utcnb_system* s, int shares, int share, const double* react_sum3)
 {
-    if (!s || !react_sum3)
-        return fail(UTCNB_EINVAL, "utcnb_share_finish: null argument");
+    if (!s)
+        return fail(UTCNB_EINVAL, "utcnb_share_finish: null handle");
     if (!s->has_state)
         return fail(UTCNB_ESTATE, "utcnb_share_finish: no state set");
     ShareGeom g;
     if (int rc = share_geom(s, shares, share, "utcnb_share_finish", &g))
         return rc;
+    if (g.count <= 0)
+        return UTCNB_OK;  // a share of padding bodies only: nothing to finish
+    if (!react_sum3)
+        return fail(UTCNB_EINVAL, "utcnb_share_finish: null react_sum3");
     if (!s->share_ipart || (size_t)g.Rs * (size_t)s->n_pad * 3 > s->share_ipart_floats)
         return fail(UTCNB_ESTATE, "utcnb_share_finish: utcnb_share_pairs of this share has not run");
-    if (g.count <= 0)
-        return UTCNB_OK;
     UTCNB_HIP(hipEventRecord(s->ev0, s->stream));
     hipError_t e = launch_sym_share_finish_mixed(s->args, shares, share, g.Rs, g.start, g.count, s->share_ipart,
                                                  react_sum3, s->snap[1 - s->cur], s->p64[0], s->v64, s->stream);
