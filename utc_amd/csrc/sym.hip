@@ -464,8 +464,8 @@ hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, 
 // Share g of G owns wave blocks [w0, w1) (the first NW % G shares one block
 // more) and evaluates, for each of them, the same sub-units as sym_wave does
 // in a full step: its own blocks and the half ring after it.  Its target-side
-// partials go to ipart (R_share rows, laid out as a full step's); its
-// reaction partials (for bodies of the wave blocks after each of its blocks,
+// partials go to ipart (R_share rows: the whole system's row layout, or the
+// share's own contiguous region of the buffer); its reaction partials (for bodies of the wave blocks after each of its blocks,
 // mostly owned by other shares) are summed per destination body into react3,
 // which a sharded run adds up over the shares (a reduce-scatter); the share
 // then adds its bodies' target-side partials to the summed reactions and
