@@ -264,6 +264,47 @@ def cube_logm16384_fwd_adjoint(U):
     return out
 
 
+def grad_fd_check(U, ncoords=64):
+    """The gradient target: the mixed adjoint of the N = 16384 cube (log-uniform
+    masses, 50 steps, checkpoint every 10) against FP64 central differences on
+    64 random coordinates (positions, velocities, masses); each perturbed loss
+    is an FP64 forward run on the GPU.  Steps as large as the loss's curvature
+    allows: 1e-6 for a position, 1e-4 for a velocity, 5% of a mass."""
+    from tests.nbcases import cube_logm16384
+    c = cube_logm16384()
+    n, steps, eps2 = len(c["pos"]), c["steps"], c["eps"] ** 2
+    t0 = time.perf_counter()
+    with U.System(n, U.PREC_MIXED, eps2, c["dt"], kick=c["kick"], flags=U.FLAG_NO_NEWTON) as s:
+        s.set_state(c["pos"], c["vel"])
+        _, dx, dv, dm = s.adjoint(steps, c["ckpt"], c["target"])
+    rng = np.random.default_rng(11)
+    kinds, bodies, comps = rng.integers(0, 3, ncoords), rng.integers(0, n, ncoords), rng.integers(0, 3, ncoords)
+    errs = []
+
+    def final(s, pos, vel):
+        s.set_state(pos, vel)
+        s.step(steps)
+        return s.get_state()[0][:, :3]
+
+    with U.System(n, U.PREC_FP64, eps2, c["dt"], kick=c["kick"]) as s:
+        for kind, i, comp in zip(kinds, bodies, comps):
+            comp = 3 if kind == 2 else comp
+            arr = c["vel"] if kind == 1 else c["pos"]
+            h = (1e-6, 1e-4, 0.05 * c["pos"][i, 3])[kind]
+            a = arr.copy(); a[i, comp] += h
+            b = arr.copy(); b[i, comp] -= h
+            pp, pm = (final(s, c["pos"], a), final(s, c["pos"], b)) if kind == 1 else \
+                     (final(s, a, c["vel"]), final(s, b, c["vel"]))
+            fd = float(((pp - pm) * (pp + pm - 2.0 * c["target"])).sum() / (n * 2.0 * h))
+            g = dm[i] if kind == 2 else (dv if kind == 1 else dx)[i, comp]
+            errs.append(abs(g - fd) / abs(fd))
+    out = {"workload": f"N={n} cube, log-uniform masses, {steps} steps, checkpoint every {c['ckpt']}, "
+                       f"{ncoords} coordinates", "max_relerr": float(max(errs)),
+           "median_relerr": float(np.median(errs)), "seconds": time.perf_counter() - t0}
+    log(f"gradient vs FP64 central differences: max rel {out['max_relerr']:.3e} ({out['seconds']:.1f} s)")
+    return out
+
+
 def fwd_adjoint(U, c, n, K, fwd_rate, ckpt=10):
     """Forward K steps with checkpoints every `ckpt`, then the adjoint of the
     mean squared distance to seeded targets back to the initial state (mixed
@@ -594,12 +635,18 @@ def main():
         log(f"reference arithmetic failed: {e}")
 
     cfg4 = None
+    fdchk = None
     if not args.no_adjoint:
         try:
             cfg4 = cube_logm16384_fwd_adjoint(U)
         except Exception as e:  # reported, never hidden
             cfg4 = {"error": str(e)}
             log(f"N=16384 adjoint leg failed: {e}")
+        try:
+            fdchk = grad_fd_check(U)
+        except Exception as e:  # reported, never hidden
+            fdchk = {"error": str(e)}
+            log(f"gradient check failed: {e}")
 
     # ---- one GPU's share of the system sharded over 8 GPUs ----------------------------
     slc = None
@@ -666,6 +713,9 @@ def main():
                                 f"{K} steps, checkpoint every 10"),
         "final_pos_relerr": target(acc["mixed"]["final_pos_relerr"], "<=", 1e-6,
                                    f"mixed path vs the FP64 path after {K} steps"),
+        "grad_fd_relerr": target(fdchk.get("max_relerr") if isinstance(fdchk, dict) else None, "<=", 1e-5,
+                                 "mixed adjoint gradients vs FP64 central differences, 64 coordinates, "
+                                 "N=16384 cube, 50 steps (max over coordinates)"),
     }
     for k, t in targets.items():
         log(f"target {k}: {t['value']} {t['target']} -> {'pass' if t['pass'] else 'MISS'}")
@@ -715,6 +765,7 @@ def main():
         "energy": energy,
         "fwd_adjoint": fwd_adj,
         "cube_logm16384_fwd_adjoint": cfg4,
+        "grad_fd_check": fdchk,
         "reference_arithmetic": ref_arith,
         "batched": batch,
         "slice_per_gpu": slc,
