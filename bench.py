@@ -454,6 +454,36 @@ def large_n(U, n=2_500_000, steps=2):
     return out
 
 
+def small_systems(U, sizes=(1024, 2048, 4096, 8192), steps=200):
+    """Small single systems, where a step is bound by latency: uniform cube,
+    equal masses 1/N, eps = 0.01, dt = 1e-3 (the N = 4096 case is the
+    reference CPU path's workload), mixed and plain FP32, the default tile
+    (one kernel per step) beside the symmetric sym2r32 (two kernels per step);
+    time per step from the wall clock over `steps` steps after a warm-up."""
+    sym = U.tiles().index("sym2r32") + 1
+    out = []
+    for n in sizes:
+        pos, vel = U.generate(U.GEN_CUBE, n, 0, 0.01)
+        for prec, label in ((U.PREC_MIXED, "mixed"), (U.PREC_FP32, "fp32")):
+            row = {"n": n, "precision": label}
+            for key, tile in (("default", 0), ("sym2r32", sym)):
+                with U.System(n, prec, 1e-4, 1e-3, flags=U.FLAG_NO_NEWTON, tile=tile) as s:
+                    s.set_state(pos, vel)
+                    s.step(20)
+                    s.sync()
+                    t0 = time.perf_counter()
+                    s.step(steps)
+                    s.sync()
+                    us = (time.perf_counter() - t0) * 1e6 / steps
+                    row[key] = {"tile": s.tile()[1], "us_per_step": us,
+                                "interactions_per_s": float(n) * n / (us * 1e-6)}
+            row["speedup_vs_sym2r32"] = row["sym2r32"]["us_per_step"] / row["default"]["us_per_step"]
+            out.append(row)
+            log(f"N={n} {label}: {row['default']['tile']} {row['default']['us_per_step']:.2f} us/step, "
+                f"sym2r32 {row['sym2r32']['us_per_step']:.2f} us/step")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -465,7 +495,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=262144,
                     help="rows of the CPU baseline's force evaluation (default: all, ~10 s on 16 threads)")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched", "reference", "slice", "large"],
+    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched", "reference", "slice", "large", "small"],
                     default=None,
                     help="time one path only and print its line (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -521,6 +551,12 @@ def main():
         r = large_n(U)
         print(json.dumps({"metric": METRIC, "value": r["interactions_per_s"], "unit": "interactions/s",
                           "n_gpus": 1, "only": "large", "large_n": r}), flush=True)
+        return
+    if args.only == "small":
+        r = small_systems(U)
+        print(json.dumps({"metric": METRIC, "value": r[0]["default"]["interactions_per_s"],
+                          "unit": "interactions/s", "n_gpus": 1, "only": "small", "small_systems": r}),
+              flush=True)
         return
     if args.only == "slice":
         r = slice_per_gpu(U, c, n, K)
@@ -663,6 +699,13 @@ def main():
         big = {"error": str(e)}
         log(f"large-N leg failed: {e}")
 
+    small = None
+    try:
+        small = small_systems(U)
+    except Exception as e:  # reported, never hidden
+        small = {"error": str(e)}
+        log(f"small-systems leg failed: {e}")
+
     # ---- 256 batched systems -------------------------------------------------------
     batch = None
     if not args.no_batch:
@@ -770,6 +813,7 @@ def main():
         "batched": batch,
         "slice_per_gpu": slc,
         "large_n": big,
+        "small_systems": small,
         "fraction_of_fp32_peak": achieved / PEAK_FP32,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_tiles():
     L = U.lib()
     assert L.utcnb_version().startswith(b"utcnb")
-    assert L.utcnb_tile_count() == 24
+    assert L.utcnb_tile_count() == 34
     assert U.tiles()[8] == "k2s8" and U.tiles()[12] == "sym8r4"
     assert L.utcnb_tile_name(0) == b"" and L.utcnb_tile_name(99) == b""
 
@@ -118,7 +118,7 @@ def test_create_validates_before_touching_a_device():
         U.Config(10, 7, 1, 1, 0, 0, 1e-4, 1.0, 1e-3),                # precision
         U.Config(10, U.PREC_MIXED, 5, 1, 0, 0, 1e-4, 1.0, 1e-3),     # kick
         U.Config(10, U.PREC_MIXED, 1, 1, 0, 0, 0.0, 1.0, 1e-3),      # eps2 = 0
-        U.Config(10, U.PREC_FP32, 1, 1, 0, 25, 1e-4, 1.0, 1e-3),     # tile
+        U.Config(10, U.PREC_FP32, 1, 1, 0, L.utcnb_tile_count() + 1, 1e-4, 1.0, 1e-3),  # tile
         U.Config(10, U.PREC_REFERENCE, 1, 0, 0, 0, 1e-4, 1.0, 1e-3), # reference needs its kick
         U.Config(10, U.PREC_FP64, 1, 1, 0, 0, 1e-4, 1.0, float("nan")),
     ]

@@ -230,13 +230,13 @@ def test_kernel_level_entry_point_f64_matches_oracle():
         assert np.array_equal(v.cpu().numpy(), ov), blocksize
 
 
-@pytest.mark.parametrize("tile", ["default", "k2s8", "sym4r32"])
+@pytest.mark.parametrize("tile", ["default", "k2s8", "sym2r32", "sym4r32"])
 def test_fp32_cube4096_within_tolerance_of_reference(tile):
     """Plain fast FP32 (rsq, FMA, uncompensated FP32 sums, FP32 state) vs the
     reference CPU path on the N=4096 cube: max |dx| <= 1e-6 and relative norm
-    <= 1e-6.  The default tile at this size (symmetric, plain FP32 block sums
-    and an FP32 reduction of the partials), a one-pass tile and another
-    symmetric one."""
+    <= 1e-6.  The default tile at this size (one kernel per step: plain FP32
+    lane sums combined in FP32), a one-pass tile and two symmetric ones
+    (plain FP32 block sums and an FP32 reduction of the partials)."""
     c = cube4096()
     pos = c["pos"].astype(np.float32)
     vel = c["vel"].astype(np.float32)
@@ -249,7 +249,7 @@ def test_fp32_cube4096_within_tolerance_of_reference(tile):
         with U.System(len(pos), U.PREC_FP32, float(eps2), float(dt), 1.0,
                       kick=U.KICK_REFERENCE, flags=flags, tile=tid) as s:
             if tile == "default":
-                assert s.tile()[1] == "sym2r32"
+                assert s.tile()[1] == "sm1t16w16n4k"
             s.set_state(pos, vel)
             s.step(c["steps"])
             gp, _ = s.get_state(np.float32)
@@ -275,6 +275,9 @@ def test_fp64_field_matches_cpu_fp64():
 @pytest.mark.parametrize("tile", list(range(1, U.lib().utcnb_tile_count() + 1)))
 def test_every_tile_field_accuracy(prec, tile):
     n = 3000  # not a multiple of any tile: exercises clamped targets and padding
+    name = U.tiles()[tile - 1]
+    if name.startswith("sm"):  # small-system tiles hold at most CAP bodies (name ends nCAPk)
+        n = min(n, int(name.split("n")[-1][:-1]) * 1024 - 47)
     c = plummer(n, eps=0.005)
     a64 = O.accel_f64(c["pos"], c["eps"] ** 2)
     say(f"case: field prec={prec} tile={tile} n={n}")
@@ -285,8 +288,18 @@ def test_every_tile_field_accuracy(prec, tile):
     assert relnorm(a, a64) <= 2e-6
 
 
+@pytest.mark.parametrize("name", [t for t in U.tiles() if t.startswith("sm")])
+def test_small_tiles_reject_systems_above_their_capacity(name):
+    cap = int(name.split("n")[-1][:-1]) * 1024
+    with U.System(cap, U.PREC_MIXED, 1e-4, 1e-3, tile=U.tiles().index(name) + 1):
+        pass
+    with pytest.raises(U.UtcnbError, match="holds at most"):
+        U.System(cap + 1, U.PREC_MIXED, 1e-4, 1e-3, tile=U.tiles().index(name) + 1)
+
+
 @pytest.mark.parametrize("pattern", ["equal", "one_heavy", "few_heavy", "loguniform"])
-@pytest.mark.parametrize("name", ["sym2r32", "sym4r32", "sym8r32", "sym8r64", "sym8r4"])
+@pytest.mark.parametrize("name", ["sym2r32", "sym4r32", "sym8r32", "sym8r64", "sym8r4", "sym2r128x4", "sym4r64x2",
+                                  "sym4r128x4"])
 def test_symmetric_tiles_mass_patterns(name, pattern):
     n = 6000  # not a multiple of 512: the last wave block holds padding
     c = plummer(n, eps=0.005)

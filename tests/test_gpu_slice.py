@@ -131,6 +131,8 @@ def test_kernel_level_slice_rejects_bad_ranges():
 CASES = [
     ("mixed", U.PREC_MIXED, "k1s16", U.F64),
     ("fp32", U.PREC_FP32, "k2s8", U.F64),
+    ("small", U.PREC_MIXED, "sm1t16w16n8k", U.F64),
+    ("small_fp32", U.PREC_FP32, "sm2t16w16n8k", U.F64),
     ("fp64", U.PREC_FP64, None, U.F64),
     ("ref_f32", U.PREC_REFERENCE, None, U.F32),
     ("ref_f64", U.PREC_REFERENCE, None, U.F64),
@@ -377,8 +379,7 @@ def test_share_calls_reject_other_systems():
     with U.System(4096, U.PREC_MIXED, 1e-4, 1e-3, tile=tid) as s:
         assert L.utcnb_share_range(s.h, 2, 0, C.byref(a), C.byref(b)) == U.EINVAL
         assert b"symmetric" in L.utcnb_last_error()
-    with U.System(4096, U.PREC_MIXED, 1e-4, 1e-3) as s:
-        assert s.tile()[1].startswith("sym")
+    with U.System(4096, U.PREC_MIXED, 1e-4, 1e-3, tile=U.tiles().index("sym2r32") + 1) as s:
         assert L.utcnb_share_range(s.h, 0, 0, C.byref(a), C.byref(b)) == U.EINVAL
         assert L.utcnb_share_range(s.h, 2, 2, C.byref(a), C.byref(b)) == U.EINVAL
         assert L.utcnb_share_range(s.h, 100000, 0, C.byref(a), C.byref(b)) == U.EINVAL

@@ -118,8 +118,8 @@ extern "C" int utcnb_autotune(int n, int precision, const char* cache_path, int*
         c.dt = 1e-4;
         utcnb_system* s = nullptr;
         if ((rc = utcnb_create(&c, nullptr, &s))) {
-            if (rc == UTCNB_EINVAL && tile_get(t - 1).sym)
-                continue;  // reaction workspace too large at this size: not a candidate
+            if (rc == UTCNB_EINVAL && (tile_get(t - 1).sym || tile_get(t - 1).cap > 0))
+                continue;  // workspace too large, or too many bodies for the tile: not a candidate
             return rc;
         }
         double ms = 0.0;

@@ -425,6 +425,12 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
         if (tile_get(t).sym)
             s->args.tile = t;
     }
+    if (fast && c.precision != UTCNB_PREC_FP64 && tile_get(s->args.tile).cap > 0 &&
+        s->n_pad > tile_get(s->args.tile).cap) {
+        delete s;
+        return fail(UTCNB_EINVAL, std::string("utcnb_create: tile ") + tile_get(s->args.tile).name + " holds at most " +
+                                      std::to_string(tile_get(s->args.tile).cap) + " bodies (N padded to 1024)");
+    }
 
     switch (c.precision) {
     case UTCNB_PREC_REFERENCE:
@@ -451,7 +457,7 @@ int utcnb_create(const utcnb_config* cfg, void* stream, utcnb_system** out)
         if (np % (size_t)(128 * t.k) != 0) {
             rc = fail(UTCNB_EINVAL, "utcnb_create: symmetric tile needs N padded to 128*K");
         } else {
-            const size_t jf = sym_jpart_floats((int)np, t.k),
+            const size_t jf = sym_jpart_floats((int)np, t.k, t.rs),
                          jif = sym_ipart_floats((int)np, t.k, sym_tile_r(t, (int)np));
             const bool full_fits = (jf + jif) * sizeof(float) <= s->partial_budget;
             if (c.precision == UTCNB_PREC_MIXED && (!full_fits || (c.flags & UTCNB_FLAG_SYM_CHUNKS))) {
@@ -975,10 +981,13 @@ static int adjoint_workspace(utcnb_system* s, int nck, int seg)
     if (!rc && f32)
         rc = dalloc(&a.wsnap[0], np) || dalloc(&a.wsnap[1], np);
     const int Rv = sym_vjp_splits((int)np, s->cu_count);
-    if (!rc && f32 && tile_get(s->args.tile).sym && sym_vjp_applies((int)np) &&
+    const Tile& ft = tile_get(s->args.tile);
+    if (!rc && f32 && (ft.sym || ft.cap > 0) && sym_vjp_applies((int)np) &&
         (sym_vjp_jpart_float4s((int)np) + sym_vjp_ipart_float4s((int)np, Rv)) * sizeof(float4) <=
             s->partial_budget) {
-        // a symmetric force tile also evaluates the adjoint pair sum symmetrically
+        // a symmetric force tile (or a small-system one, whose sizes the
+        // symmetric VJP also serves best) evaluates the adjoint pair sum
+        // symmetrically
         const int R = Rv;
         rc = dalloc(&a.vjp_ipart, sym_vjp_ipart_float4s((int)np, R)) ||
              dalloc(&a.vjp_jpart, sym_vjp_jpart_float4s((int)np));

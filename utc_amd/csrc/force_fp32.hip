@@ -18,28 +18,45 @@ const Tile kTiles[] = {
     {4, 32, "sym4r32", true}, {8, 32, "sym8r32", true}, {4, 64, "sym4r64", true},
     {4, 128, "sym4r128", true}, {2, 32, "sym2r32", true}, {2, 64, "sym2r64", true},
     {8, 64, "sym8r64", true}, {8, 128, "sym8r128", true},
+    // symmetric with each sub-unit's rotation cut into RS chunks: symKrRxRS
+    {2, 64, "sym2r64x2", true, 2}, {4, 64, "sym4r64x2", true, 2}, {2, 128, "sym2r128x4", true, 4},
+    {4, 128, "sym4r128x4", true, 4},
+    // one kernel per step for small systems (force_small.h): A targets per
+    // lane, TL target lanes per wave, W waves, at most CAP bodies:
+    // smAtTLwWnCAPk; k = A, s = W
+    {1, 8, "sm1t4w8n2k", false, 1, 4, 2048}, {1, 8, "sm1t8w8n2k", false, 1, 8, 2048},
+    {1, 16, "sm1t8w16n8k", false, 1, 8, 8192}, {1, 16, "sm1t16w16n4k", false, 1, 16, 4096},
+    {1, 16, "sm1t16w16n8k", false, 1, 16, 8192}, {2, 16, "sm2t16w16n8k", false, 1, 16, 8192},
 };
 }  // namespace
 
 int tile_count() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 const Tile& tile_get(int id) { return kTiles[id]; }
 
-// Default tile, from sweeps of every tile on MI355X (mixed precision, ms/step,
-// best three; Plummer spheres, equal masses; log-uniform masses rank alike):
-//   N = 262144: sym8r64 12.29, sym8r48 12.34, sym8r32 12.34   (k2s8, one-pass: 20.7)
-//   N = 131072: sym8r48 3.139, sym8r64 3.143, sym8r32 3.160
-//   N =  65536: sym8r64 0.830, sym8r48 0.830, sym8r32 0.871
-//   N =  32768: sym8r64 0.235, sym8r48 0.239, sym8r32 0.240
-//   N =  16384: sym4r64 0.0758, sym4r32 0.0775, sym4r48 0.0783
-//   N =   8192: sym4r128 0.0321, sym2r32 0.0324, sym4r64 0.0327
-//   N =   4096: sym2r32 0.0170, sym2r64 0.0170, sym4r64 0.0180  (k1s16: 0.0315)
-//   N =   2048: sym2r32 0.0116, sym2r64 0.0128, sym4r32 0.0157  (k1s16: 0.0173)
-//   N =   1024: k1s16 0.0101, sym2r32 0.0112
-// (sym8r48 and sym8r24, tried for a whole number of wave rounds, were not
-// faster and are not in the table.)
-// The symmetric tiles need enough wave blocks to fill the chip (K = 8 from 128
-// bodies per CU, K = 4 from 32), and their partial sums must fit the device's
-// budget (sym_partial_budget); above it the best one-pass tile (k2s8) is used.
+// Default tile, from sweeps of every tile on MI355X (mixed precision, time per
+// step, best three; Plummer spheres, equal masses; log-uniform masses rank
+// alike):
+//   N = 262144: sym8r64 12.29, sym8r48 12.34, sym8r32 12.34 ms  (k2s8, one-pass: 20.7)
+//   N = 131072: sym8r48 3.139, sym8r64 3.143, sym8r32 3.160 ms
+//   N =  65536: sym8r64 0.830, sym8r48 0.830, sym8r32 0.871 ms
+//   N =  32768: sym8r64 0.235, sym8r48 0.239, sym8r32 0.240 ms
+//   N =  24576: sym8r64 148.0, sym4r32 150.9, sym4r64 152.5 us
+//   N =  16384: sym4r64x2 75.9, sym4r64 77.5, sym4r128x2 79.3 us
+//   N =  12288: sym2r32 51.8, sym4r64x2 51.8, sym4r128x2 52.4 us
+//   N =   8192: sm2t16w16n8k 28.4, sm1t16w16n8k 29.0, sm4t8w16n8k 29.5 us  (sym2r32 29.7)
+//   N =   6144: sm2t16w16n8k 21.4, sm1t8w16n8k 21.8, sm1t16w16n8k 22.4 us  (sym2r64x2 23.2)
+//   N =   4096: sm1t16w16n4k 9.57, sm1t16w16n8k 9.59, sm2t8w16n8k 10.2 us  (sym2r32 16.0)
+//   N =   3072: sm1t16w16n4k 7.88, sm1t16w16n8k 7.91, sm2t8w16n8k 8.42 us  (sym2r32 12.1)
+//   N =   2048: sm1t8w16n8k 5.11, sm1t8w16n4k 5.44, sm1t8w8n2k 5.48 us  (sym2r32 10.9)
+//   N =   1024: sm1t4w8n2k 3.94, sm1t8w8n2k 4.01, sm1t8w16n8k 4.17 us  (k1s16 10.1)
+// (sym8r48, sym8r24 and several other small-system shapes were swept and
+// are not in the table.)  Up to 8192 bodies the small-system kernels win:
+// one launch per step against two, and their one-pass pair sums fill the
+// chip where the symmetric kernels' 64-step rotations leave it waiting.  The
+// symmetric tiles need enough wave blocks to fill the chip (K = 8 from 96
+// bodies per CU, K = 4 from 64), and their partial sums must fit the
+// device's budget (sym_partial_budget); above it the best one-pass tile
+// (k2s8) is used.
 size_t sym_partial_budget(size_t device_bytes)
 {
     if (device_bytes == 0)
@@ -56,14 +73,19 @@ int tile_default(int64_t n, int cu_count, size_t partial_budget)
                (sym_jpart_floats((int)n_pad, k) + sym_ipart_floats((int)n_pad, k, 64)) * sizeof(float) <=
                    partial_budget;
     };
-    // from a sweep of every tile at N = 1024 ... 262144 (equal and log-uniform
-    // masses) on MI355X: each choice within 2% of the fastest tile at its size
-    if (n >= 128 * cus && fits(8))
+    // from the sweeps above: each choice within 2% of the fastest tile at its size
+    if (n <= 1024)
+        return 28;  // sm1t4w8n2k
+    if (n <= 2048)
+        return 30;  // sm1t8w16n8k
+    if (n <= 4096)
+        return 31;  // sm1t16w16n4k
+    if (n <= 8192)
+        return 33;  // sm2t16w16n8k
+    if (n >= 96 * cus && fits(8))
         return 22;  // sym8r64
     if (n >= 64 * cus && fits(4))
-        return 18;  // sym4r64
-    if (n >= 32 * cus && fits(4))
-        return 19;  // sym4r128
+        return 25;  // sym4r64x2
     if (n >= 8 * cus && fits(2))
         return 20;  // sym2r32
     if ((n + 127) / 128 * 8 >= cus * 16)

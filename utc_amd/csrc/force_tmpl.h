@@ -102,7 +102,9 @@ __device__ __forceinline__ void stream_sources(const float4* __restrict__ src, i
 
 // ---- epilogues ---------------------------------------------------------------
 // Called once per target i < n with the summed field a = hi + c' where c' is
-// the compensation term (zero for plain sums).
+// the compensation term (zero for plain sums).  load(i) reads the state the
+// update needs and store(i, state, ...) applies it, so a kernel can issue the
+// loads early; operator() does both.
 
 // Plain FP32 state, the reference's damped kick-drift
 // (benchapps/nbody/seq/bodysystem.cc:161-172) with kick factor pos.w or 1.
@@ -112,11 +114,15 @@ struct EpiFP32 {
     float4* vel;
     float dt, damping;
     int kick;
-    __device__ __forceinline__ void operator()(int i, float hx, float hy, float hz, float,
-                                               float, float) const
+    struct State {
+        float4 p, v;
+    };
+    __device__ __forceinline__ State load(int i) const { return {pos_old[i], vel[i]}; }
+    __device__ __forceinline__ void store(int i, State st, float hx, float hy, float hz, float, float,
+                                          float) const
     {
-        float4 p = pos_old[i];
-        float4 v = vel[i];
+        float4 p = st.p;
+        float4 v = st.v;
         float w = kick == UTCNB_KICK_REFERENCE ? p.w : 1.0f;
         v.x = (v.x + (hx * w) * dt) * damping;
         v.y = (v.y + (hy * w) * dt) * damping;
@@ -127,6 +133,11 @@ struct EpiFP32 {
         pos_new[i] = p;
         vel[i] = v;
     }
+    __device__ __forceinline__ void operator()(int i, float hx, float hy, float hz, float cx, float cy,
+                                               float cz) const
+    {
+        store(i, load(i), hx, hy, hz, cx, cy, cz);
+    }
 };
 
 // FP64 state updated in place, new FP32 snapshot written for the next step.
@@ -136,15 +147,19 @@ struct EpiMixed {
     float4* snap_new;
     double dt, damping;
     int kick;
-    __device__ __forceinline__ void operator()(int i, float hx, float hy, float hz, float cx,
-                                               float cy, float cz) const
+    struct State {
+        double4 p, v;
+    };
+    __device__ __forceinline__ State load(int i) const { return {pos[i], vel[i]}; }
+    __device__ __forceinline__ void store(int i, State st, float hx, float hy, float hz, float cx, float cy,
+                                          float cz) const
     {
         // Kahan keeps the running error as -c; the sum is hi - c.
         double ax = (double)hx - (double)cx;
         double ay = (double)hy - (double)cy;
         double az = (double)hz - (double)cz;
-        double4 p = pos[i];
-        double4 v = vel[i];
+        double4 p = st.p;
+        double4 v = st.v;
         double w = kick == UTCNB_KICK_REFERENCE ? p.w : 1.0;
         v.x = (v.x + (ax * w) * dt) * damping;
         v.y = (v.y + (ay * w) * dt) * damping;
@@ -156,6 +171,11 @@ struct EpiMixed {
         vel[i] = v;
         snap_new[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
     }
+    __device__ __forceinline__ void operator()(int i, float hx, float hy, float hz, float cx, float cy,
+                                               float cz) const
+    {
+        store(i, load(i), hx, hy, hz, cx, cy, cz);
+    }
 };
 
 // Field only, as FP64 [3n]; body i goes to row i - off (off = the first
@@ -163,15 +183,28 @@ struct EpiMixed {
 struct EpiAccel {
     double* acc3;
     int off = 0;
-    __device__ __forceinline__ void operator()(int i, float hx, float hy, float hz, float cx,
-                                               float cy, float cz) const
+    struct State {};
+    __device__ __forceinline__ State load(int) const { return {}; }
+    __device__ __forceinline__ void store(int i, State, float hx, float hy, float hz, float cx, float cy,
+                                          float cz) const
     {
         const size_t r = 3 * (size_t)(i - off);
         acc3[r + 0] = (double)hx - (double)cx;
         acc3[r + 1] = (double)hy - (double)cy;
         acc3[r + 2] = (double)hz - (double)cz;
     }
+    __device__ __forceinline__ void operator()(int i, float hx, float hy, float hz, float cx, float cy,
+                                               float cz) const
+    {
+        store(i, load(i), hx, hy, hz, cx, cy, cz);
+    }
 };
+
+}  // namespace utcnb
+
+#include "force_small.h"  // uses pair_scale and kahan_add above
+
+namespace utcnb {
 
 // ---- the kernel --------------------------------------------------------------
 // W waves per workgroup; S of them split the sources of one target tile, so a
@@ -275,6 +308,12 @@ hipError_t launch_force_range(int tile, const float4* src, int i0, int nt, int n
     case 7: return launch_force_tile<1, 8, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
     case 8: return launch_force_tile<2, 8, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
     case 9: return launch_force_tile<1, 16, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 28: return launch_small_tile<1, 4, 8, 2048, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 29: return launch_small_tile<1, 8, 8, 2048, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 30: return launch_small_tile<1, 8, 16, 8192, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 31: return launch_small_tile<1, 16, 16, 4096, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 32: return launch_small_tile<1, 16, 16, 8192, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 33: return launch_small_tile<2, 16, 16, 8192, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
     default: return hipErrorInvalidValue;
     }
 }

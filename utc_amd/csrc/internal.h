@@ -38,6 +38,9 @@ struct Tile {
     int s;       // source split (one-pass tiles) or waves per wave block (symmetric tiles)
     const char* name;
     bool sym;    // evaluates each unordered pair once (sym.hip)
+    int rs = 1;  // symmetric tiles: rotation chunks per sub-unit (1, 2, 4, 8 or 16; K <= 4 when > 1)
+    int tl = 0;  // small-system tiles (force_small.h): target lanes per wave; 0 for the other tiles
+    int cap = 0; // small-system tiles: the most bodies (padded) their LDS holds
 };
 int tile_count();
 const Tile& tile_get(int id);
@@ -71,7 +74,7 @@ struct StepArgs {
 int sym_tile_r(const Tile& t, int n_pad);
 // Workspace of a symmetric tile (floats).
 size_t sym_ipart_floats(int n_pad, int K, int R);
-size_t sym_jpart_floats(int n_pad, int K);
+size_t sym_jpart_floats(int n_pad, int K, int rs = 1);
 
 // Reference arithmetic, one fused step (pos_new/vel written), T = float/double.
 hipError_t launch_ref_step_f32(const float4* pos_old, float4* pos_new, float4* vel, int n,

@@ -118,7 +118,7 @@ extern "C" int utcnb_accel_noalloc(const float* pos4, int n, double eps2, int pr
             ipart_off = need;
             need += round256(sym_ipart_floats((int)n_pad, tl.k, sym_tile_r(tl, (int)n_pad)) * sizeof(float));
             jpart_off = need;
-            need += round256(sym_jpart_floats((int)n_pad, tl.k) * sizeof(float));
+            need += round256(sym_jpart_floats((int)n_pad, tl.k, tl.rs) * sizeof(float));
         }
         return need;
     };
@@ -128,6 +128,9 @@ extern "C" int utcnb_accel_noalloc(const float* pos4, int n, double eps2, int pr
     const Tile& tl = tile_get(t);
     if (tl.sym && n_pad % (128 * tl.k) != 0)
         return fail(UTCNB_EINVAL, "utcnb_accel_noalloc: symmetric tile needs N padded to 128*K");
+    if (tl.cap > 0 && n_pad > tl.cap)
+        return fail(UTCNB_EINVAL, std::string("utcnb_accel_noalloc: tile ") + tl.name + " holds at most " +
+                                      std::to_string(tl.cap) + " bodies (N padded to 1024)");
     const size_t need = need_of(t);
     if (need > kScratchBytes)
         return fail(UTCNB_EINVAL, "utcnb_accel_noalloc: n = " + std::to_string(n) + " with tile " + tl.name +

@@ -29,6 +29,14 @@
 // Q = K (NW/2 + 1).  The waves' target-side partials and the reaction
 // partials go to HBM, and a second kernel adds them per body in a fixed order
 // (deterministic) and applies the integrator epilogue.
+// Small systems have too few sub-units to give every SIMD several waves, and
+// one wave alone on a SIMD waits on its own dependent instructions.  Tiles
+// with a rotation split RS (SPLIT kernels) cut each sub-unit's 64 rotation
+// steps into RS chunks of 64/RS steps, and the R waves of a wave block share
+// its Q * RS chunks: chunk c starts with the source block already rotated by
+// 64c/RS lanes (lane l loads source (l - 64c/RS) mod 64) and leaves the
+// reaction sums rotated by 64(c+1)/RS, so each chunk stores its reactions to
+// the source lane they belong to, in a reaction record of its own.
 #include "force_tmpl.h"
 #include "sym_items.h"
 #include "sym_reduce.h"
@@ -60,17 +68,41 @@ __device__ __forceinline__ void block_add(float& a, float& c, float p)
         a += p;
 }
 
+// One rotation step of sym_block (statement macro: the same code in the
+// fixed 64-step loop and in the chunked loop).
+#define SYM_BLOCK_STEP()                                                                               \
+    _Pragma("unroll")                                                                                  \
+    for (int k = 0; k < K; ++k) {                                                                      \
+        const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];                                 \
+        const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2))); \
+        const float y3 = pair_scale<NEWTON>(d2, 1.0f);                                                 \
+        const float si = UNI ? y3 : sm * y3;                                                           \
+        const float cj = UNI ? y3 : mi[k] * y3;                                                        \
+        px[k] = __builtin_fmaf(rx, si, px[k]);                                                         \
+        py[k] = __builtin_fmaf(ry, si, py[k]);                                                         \
+        pz[k] = __builtin_fmaf(rz, si, pz[k]);                                                         \
+        gx = __builtin_fmaf(-rx, cj, gx);                                                              \
+        gy = __builtin_fmaf(-ry, cj, gy);                                                              \
+        gz = __builtin_fmaf(-rz, cj, gz);                                                              \
+    }                                                                                                  \
+    sx = wave_ror1<K>(sx); sy = wave_ror1<K>(sy); sz = wave_ror1<K>(sz);                               \
+    if constexpr (!UNI)                                                                                \
+        sm = wave_ror1<K>(sm);                                                                         \
+    gx = wave_ror1<K>(gx); gy = wave_ror1<K>(gy); gz = wave_ror1<K>(gz);
+
 // A block of 64 sources (one per lane, s) against the K targets of each lane,
 // both sides: the target-side block partials are added into (a, c) by
 // block_add<COMP>, the reaction sums of the lane's source come back in g after
 // 64 rotations.
 // UNI: every target mass mi[k] and source mass equals m, so the pair terms
 // drop the mass factors and the block sums are scaled by m at the end.
-template <int K, bool NEWTON, bool COMP, bool UNI>
+// SPLIT: `steps` rotation steps (a chunk of the rotation), else all 64.
+template <int K, bool NEWTON, bool COMP, bool UNI, bool SPLIT = false>
 __device__ __forceinline__ void sym_block(float4 s, const float (&xi)[K], const float (&yi)[K],
                                           const float (&zi)[K], const float (&mi)[K], float m, float eps2,
                                           float (&ax)[K], float (&ay)[K], float (&az)[K], float (&cx)[K],
-                                          float (&cy)[K], float (&cz)[K], float& gx, float& gy, float& gz)
+                                          float (&cy)[K], float (&cz)[K], float& gx, float& gy, float& gz,
+                                          int steps = 64)
 {
     float sx = s.x, sy = s.y, sz = s.z, sm = s.w;
     float px[K], py[K], pz[K];
@@ -78,26 +110,19 @@ __device__ __forceinline__ void sym_block(float4 s, const float (&xi)[K], const 
     for (int k = 0; k < K; ++k)
         px[k] = py[k] = pz[k] = 0.f;
     gx = gy = gz = 0.f;
-#pragma unroll 4
-    for (int t = 0; t < 64; ++t) {
+    if constexpr (SPLIT) {
+        // a chunk's `steps` (a multiple of 4) in unrolled groups of 4: a loop
+        // of DPP steps with a run-time trip count is not unrolled otherwise
+        for (int t = 0; t < steps; t += 4)
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
-            const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-            const float y3 = pair_scale<NEWTON>(d2, 1.0f);
-            const float si = UNI ? y3 : sm * y3;
-            const float cj = UNI ? y3 : mi[k] * y3;
-            px[k] = __builtin_fmaf(rx, si, px[k]);
-            py[k] = __builtin_fmaf(ry, si, py[k]);
-            pz[k] = __builtin_fmaf(rz, si, pz[k]);
-            gx = __builtin_fmaf(-rx, cj, gx);
-            gy = __builtin_fmaf(-ry, cj, gy);
-            gz = __builtin_fmaf(-rz, cj, gz);
+            for (int t4 = 0; t4 < 4; ++t4) {
+                SYM_BLOCK_STEP();
+            }
+    } else {
+#pragma unroll 4
+        for (int t = 0; t < 64; ++t) {
+            SYM_BLOCK_STEP();
         }
-        sx = wave_ror1<K>(sx); sy = wave_ror1<K>(sy); sz = wave_ror1<K>(sz);
-        if constexpr (!UNI)
-            sm = wave_ror1<K>(sm);
-        gx = wave_ror1<K>(gx); gy = wave_ror1<K>(gy); gz = wave_ror1<K>(gz);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -113,17 +138,51 @@ __device__ __forceinline__ void sym_block(float4 s, const float (&xi)[K], const 
     }
 }
 
+// One rotation step of sym_own.
+#define SYM_OWN_STEP()                                                                                 \
+    _Pragma("unroll")                                                                                  \
+    for (int k = 0; k < KB; ++k) {                                                                     \
+        const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];                                 \
+        const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2))); \
+        const float y3 = pair_scale<NEWTON>(d2, 1.0f);                                                 \
+        const float si = UNI ? y3 : sm * y3;                                                           \
+        const float cj = UNI ? y3 : mi[k] * y3;                                                        \
+        px[k] = __builtin_fmaf(rx, si, px[k]);                                                         \
+        py[k] = __builtin_fmaf(ry, si, py[k]);                                                         \
+        pz[k] = __builtin_fmaf(rz, si, pz[k]);                                                         \
+        gx = __builtin_fmaf(-rx, cj, gx);                                                              \
+        gy = __builtin_fmaf(-ry, cj, gy);                                                              \
+        gz = __builtin_fmaf(-rz, cj, gz);                                                              \
+    }                                                                                                  \
+    {                                                                                                  \
+        const float rx = sx - xi[KB], ry = sy - yi[KB], rz = sz - zi[KB];                              \
+        const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2))); \
+        const float si = pair_scale<NEWTON>(d2, UNI ? 1.0f : sm);                                      \
+        px[KB] = __builtin_fmaf(rx, si, px[KB]);                                                       \
+        py[KB] = __builtin_fmaf(ry, si, py[KB]);                                                       \
+        pz[KB] = __builtin_fmaf(rz, si, pz[KB]);                                                       \
+    }                                                                                                  \
+    sx = wave_ror1<K>(sx); sy = wave_ror1<K>(sy); sz = wave_ror1<K>(sz);                               \
+    if constexpr (!UNI)                                                                                \
+        sm = wave_ror1<K>(sm);                                                                         \
+    if constexpr (KB > 0) {                                                                            \
+        gx = wave_ror1<K>(gx); gy = wave_ror1<K>(gy); gz = wave_ror1<K>(gz);                           \
+    }
+
 // Own source block KB of the wave's block (K <= 4): target slots k < KB meet
 // it two-sided (the pairs of blocks k and KB, each unordered pair once), slot
 // KB one-sided (the block with itself: each ordered pair once, the self pair
 // adds r = 0); slots k > KB take these sources when their own block is the
 // source.  The reaction sums come home to slot KB after 64 rotations.  UNI as
-// in sym_block.
-template <int K, bool NEWTON, bool COMP, bool UNI, int KB>
+// in sym_block.  SPLIT: a chunk of `steps` rotation steps that ends `rot`
+// lanes short of home (lane l then holds the reactions of lane l - rot's
+// body); the sums are moved home by a lane shuffle.
+template <int K, bool NEWTON, bool COMP, bool UNI, int KB, bool SPLIT = false>
 __device__ __forceinline__ void sym_own(float4 s, const float (&xi)[K], const float (&yi)[K],
                                         const float (&zi)[K], const float (&mi)[K], float m, float eps2,
                                         float (&ax)[K], float (&ay)[K], float (&az)[K], float (&cx)[K],
-                                        float (&cy)[K], float (&cz)[K])
+                                        float (&cy)[K], float (&cz)[K], int steps = 64, int rot = 0,
+                                        int lane = 0)
 {
     float sx = s.x, sy = s.y, sz = s.z, sm = s.w;
     float px[KB + 1], py[KB + 1], pz[KB + 1];
@@ -131,35 +190,16 @@ __device__ __forceinline__ void sym_own(float4 s, const float (&xi)[K], const fl
     for (int k = 0; k <= KB; ++k)
         px[k] = py[k] = pz[k] = 0.f;
     float gx = 0.f, gy = 0.f, gz = 0.f;
-#pragma unroll 4
-    for (int t = 0; t < 64; ++t) {
+    if constexpr (SPLIT) {
+        for (int t = 0; t < steps; t += 4)  // in unrolled groups of 4, as in sym_block
 #pragma unroll
-        for (int k = 0; k < KB; ++k) {
-            const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
-            const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-            const float y3 = pair_scale<NEWTON>(d2, 1.0f);
-            const float si = UNI ? y3 : sm * y3;
-            const float cj = UNI ? y3 : mi[k] * y3;
-            px[k] = __builtin_fmaf(rx, si, px[k]);
-            py[k] = __builtin_fmaf(ry, si, py[k]);
-            pz[k] = __builtin_fmaf(rz, si, pz[k]);
-            gx = __builtin_fmaf(-rx, cj, gx);
-            gy = __builtin_fmaf(-ry, cj, gy);
-            gz = __builtin_fmaf(-rz, cj, gz);
-        }
-        {
-            const float rx = sx - xi[KB], ry = sy - yi[KB], rz = sz - zi[KB];
-            const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-            const float si = pair_scale<NEWTON>(d2, UNI ? 1.0f : sm);
-            px[KB] = __builtin_fmaf(rx, si, px[KB]);
-            py[KB] = __builtin_fmaf(ry, si, py[KB]);
-            pz[KB] = __builtin_fmaf(rz, si, pz[KB]);
-        }
-        sx = wave_ror1<K>(sx); sy = wave_ror1<K>(sy); sz = wave_ror1<K>(sz);
-        if constexpr (!UNI)
-            sm = wave_ror1<K>(sm);
-        if constexpr (KB > 0) {
-            gx = wave_ror1<K>(gx); gy = wave_ror1<K>(gy); gz = wave_ror1<K>(gz);
+            for (int t4 = 0; t4 < 4; ++t4) {
+                SYM_OWN_STEP();
+            }
+    } else {
+#pragma unroll 4
+        for (int t = 0; t < 64; ++t) {
+            SYM_OWN_STEP();
         }
     }
 #pragma unroll
@@ -172,6 +212,12 @@ __device__ __forceinline__ void sym_own(float4 s, const float (&xi)[K], const fl
         block_add<COMP>(az[k], cz[k], pz[k]);
     }
     if constexpr (KB > 0) {
+        if constexpr (SPLIT) {
+            if (rot != 0) {  // wave-uniform
+                const int from = (lane + rot) & 63;
+                gx = __shfl(gx, from, 64); gy = __shfl(gy, from, 64); gz = __shfl(gz, from, 64);
+            }
+        }
         if constexpr (UNI) {
             gx *= m; gy *= m; gz *= m;
         }
@@ -182,19 +228,23 @@ __device__ __forceinline__ void sym_own(float4 s, const float (&xi)[K], const fl
 }
 
 // sym_own for the wave-uniform block index kb, 0 <= kb < K.
-template <int K, bool NEWTON, bool COMP, int KB = 0>
+template <int K, bool NEWTON, bool COMP, bool SPLIT = false, int KB = 0>
 __device__ __forceinline__ void sym_own_at(int kb, bool uni, float4 s, const float (&xi)[K], const float (&yi)[K],
                                            const float (&zi)[K], const float (&mi)[K], float m, float eps2,
                                            float (&ax)[K], float (&ay)[K], float (&az)[K], float (&cx)[K],
-                                           float (&cy)[K], float (&cz)[K])
+                                           float (&cy)[K], float (&cz)[K], int steps = 64, int rot = 0,
+                                           int lane = 0)
 {
     if (kb == KB) {
         if (uni)
-            sym_own<K, NEWTON, COMP, true, KB>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
+            sym_own<K, NEWTON, COMP, true, KB, SPLIT>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz, steps,
+                                                      rot, lane);
         else
-            sym_own<K, NEWTON, COMP, false, KB>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
+            sym_own<K, NEWTON, COMP, false, KB, SPLIT>(s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz, steps,
+                                                       rot, lane);
     } else if constexpr (KB + 1 < K) {
-        sym_own_at<K, NEWTON, COMP, KB + 1>(kb, uni, s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz);
+        sym_own_at<K, NEWTON, COMP, SPLIT, KB + 1>(kb, uni, s, xi, yi, zi, mi, m, eps2, ax, ay, az, cx, cy, cz,
+                                                   steps, rot, lane);
     }
 }
 
@@ -205,11 +255,14 @@ __device__ __forceinline__ void sym_own_at(int kb, bool uni, float4 s, const flo
 // range of bodies only: body b of row r is record r * istride + b - ibody0
 // (istride < 0: n_pad, the whole system).  COMP: the target-side block
 // partials are Kahan-added (the compensated paths); otherwise plain FP32 sums.
-template <int K, bool NEWTON, bool COMP>
+// SPLIT: every sub-unit is cut into rs rotation chunks (rs divides 64); the
+// reactions then take rs records per (wave block, unit): jpart is
+// [NW][NW/2][rs][64K][3].
+template <int K, bool NEWTON, bool COMP, bool SPLIT = false>
 __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW, int R, int w, int r,
                                          float eps2, float* __restrict__ ipart,
                                          float* __restrict__ jpart, int lane, int jbase = 0,
-                                         long long istride = -1, int ibody0 = 0)
+                                         long long istride = -1, int ibody0 = 0, int rs = 1)
 {
     const int half = NW / 2;
     const int n_pad = NW * 64 * K;
@@ -229,48 +282,64 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
     // sub-unit q = (unit u, source block kb): u = 0 is the wave block's own
     // sources, u = 1 + d the wave block d + 1 places round
     const int Q = (half + 1) * K;
-    const int q0 = (int)((long long)Q * r / R), q1 = (int)((long long)Q * (r + 1) / R);
-    for (int q = q0; q < q1; ++q) {
+    const int QS = SPLIT ? Q * rs : Q;  // (sub-unit, chunk) pairs, chunk fastest
+    const int steps = SPLIT ? 64 / rs : 64;
+    const int q0 = (int)((long long)QS * r / R), q1 = (int)((long long)QS * (r + 1) / R);
+    for (int qs = q0; qs < q1; ++qs) {
+        int q = qs, c = 0;
+        if constexpr (SPLIT) {
+            q = qs / rs;
+            c = qs - q * rs;
+        }
         const int u = q / K, kb = q - u * K;
+        // chunk c runs rotation steps [t0, t0 + steps): lane l starts with
+        // source (l - t0) mod 64 and ends holding source (l - t0 - steps) mod 64
+        const int t0 = c * steps;
+        const int sl = SPLIT ? ((lane - t0) & 63) : lane;
+        const int el = SPLIT ? ((lane - t0 - steps) & 63) : lane;
         if constexpr (K <= 4) {
             if (u == 0) {
                 // own block kb: two-sided against the lower target slots
-                const float4 s = src[w * 64 * K + kb * 64 + lane];
+                const float4 s = src[w * 64 * K + kb * 64 + sl];
                 const bool uni = __ballot(tuni && s.w == m0) == ~0ull;
-                sym_own_at<K, NEWTON, COMP>(kb, uni, s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz);
+                sym_own_at<K, NEWTON, COMP, SPLIT>(kb, uni, s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz,
+                                                   steps, (t0 + steps) & 63, lane);
                 continue;
             }
         }
-        if (u == 0) {
-            // own block kb (K = 8, where own blocks are a small share and
-            // eight variants of sym_own would crowd the instruction cache):
-            // target side only (each ordered pair once, self pair gives 0)
-            const float4 s = src[w * 64 * K + kb * 64 + lane];
-            float sx = s.x, sy = s.y, sz = s.z, sm = s.w;
-            float px[K], py[K], pz[K];
+        if constexpr (K > 4) {
+            if (u == 0) {
+                // own block kb (K = 8, where own blocks are a small share and
+                // eight variants of sym_own would crowd the instruction cache):
+                // target side only (each ordered pair once, self pair gives 0)
+                static_assert(!SPLIT, "rotation chunks need K <= 4");
+                const float4 s = src[w * 64 * K + kb * 64 + lane];
+                float sx = s.x, sy = s.y, sz = s.z, sm = s.w;
+                float px[K], py[K], pz[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                px[k] = py[k] = pz[k] = 0.f;
+                for (int k = 0; k < K; ++k)
+                    px[k] = py[k] = pz[k] = 0.f;
 #pragma unroll 4
-            for (int t = 0; t < 64; ++t) {
+                for (int t = 0; t < 64; ++t) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
+                        const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+                        const float si = pair_scale<NEWTON>(d2, sm);
+                        px[k] = __builtin_fmaf(rx, si, px[k]);
+                        py[k] = __builtin_fmaf(ry, si, py[k]);
+                        pz[k] = __builtin_fmaf(rz, si, pz[k]);
+                    }
+                    sx = wave_ror1<K>(sx); sy = wave_ror1<K>(sy); sz = wave_ror1<K>(sz); sm = wave_ror1<K>(sm);
+                }
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const float rx = sx - xi[k], ry = sy - yi[k], rz = sz - zi[k];
-                    const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-                    const float si = pair_scale<NEWTON>(d2, sm);
-                    px[k] = __builtin_fmaf(rx, si, px[k]);
-                    py[k] = __builtin_fmaf(ry, si, py[k]);
-                    pz[k] = __builtin_fmaf(rz, si, pz[k]);
+                    block_add<COMP>(ax[k], cx[k], px[k]);
+                    block_add<COMP>(ay[k], cy[k], py[k]);
+                    block_add<COMP>(az[k], cz[k], pz[k]);
                 }
-                sx = wave_ror1<K>(sx); sy = wave_ror1<K>(sy); sz = wave_ror1<K>(sz); sm = wave_ror1<K>(sm);
+                continue;
             }
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                block_add<COMP>(ax[k], cx[k], px[k]);
-                block_add<COMP>(ay[k], cy[k], py[k]);
-                block_add<COMP>(az[k], cz[k], pz[k]);
-            }
-            continue;
         }
         const int d = u - 1;
         if (2 * (d + 1) == NW && w >= half)
@@ -278,13 +347,15 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
         int v = w + 1 + d;
         if (v >= NW)
             v -= NW;
-        const float4 s = src[v * 64 * K + kb * 64 + lane];
+        const float4 s = src[v * 64 * K + kb * 64 + sl];
         float gx, gy, gz;
         if (__ballot(tuni && s.w == m0) == ~0ull)
-            sym_block<K, NEWTON, COMP, true>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
+            sym_block<K, NEWTON, COMP, true, SPLIT>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz,
+                                                    steps);
         else
-            sym_block<K, NEWTON, COMP, false>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy, gz);
-        const size_t o = (((size_t)(w - jbase) * half + d) * (64 * K) + kb * 64 + lane) * 3;
+            sym_block<K, NEWTON, COMP, false, SPLIT>(s, xi, yi, zi, mi, m0, eps2, ax, ay, az, cx, cy, cz, gx, gy,
+                                                     gz, steps);
+        const size_t o = ((((size_t)(w - jbase) * half + d) * (SPLIT ? rs : 1) + c) * (64 * K) + kb * 64 + el) * 3;
         jpart[o + 0] = gx;
         jpart[o + 1] = gy;
         jpart[o + 2] = gz;
@@ -299,16 +370,17 @@ __device__ __forceinline__ void sym_wave(const float4* __restrict__ src, int NW,
     }
 }
 
-// One system: wave gw = w * R + r.
-template <int K, bool NEWTON, bool COMP>
+// One system: wave gw = w * R + r.  SPLIT: rs rotation chunks per sub-unit.
+template <int K, bool NEWTON, bool COMP, bool SPLIT>
 __global__ __launch_bounds__(256) void sym_force_kernel(const float4* __restrict__ src, int NW, int R,
                                                         float eps2, float* __restrict__ ipart,
-                                                        float* __restrict__ jpart)
+                                                        float* __restrict__ jpart, int rs)
 {
     const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (gw >= NW * R)
         return;  // whole waves only
-    sym_wave<K, NEWTON, COMP>(src, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63);
+    sym_wave<K, NEWTON, COMP, SPLIT>(src, NW, R, gw / R, gw % R, eps2, ipart, jpart, threadIdx.x & 63, 0, -1, 0,
+                                     rs);
 }
 
 // Batched systems: each wave runs items (system, wave block, split), a system
@@ -359,12 +431,12 @@ __device__ __forceinline__ void sym_epilogue(const Epi& epi, int i, const float 
 template <int T, typename Acc, class Epi>
 __global__ __launch_bounds__(256) void sym_reduce_kernel(const float* __restrict__ ipart,
                                                          const float* __restrict__ jpart, int n, int NW,
-                                                         int R, int K, Epi epi)
+                                                         int R, int K, int rsl, Epi epi)
 {
     int b, t;
     sym_reduce_lane<T>(b, t);
     Acc s[3];
-    sym_split_sum<T>((const Part3*)ipart, (const Part3*)jpart, b < n ? b : n - 1, NW, R, K, t, s);
+    sym_split_sum<T>((const Part3*)ipart, (const Part3*)jpart, b < n ? b : n - 1, NW, R, K, t, s, rsl);
     if (b < n && t == 0)
         sym_epilogue(epi, b, s);
 }
@@ -395,34 +467,58 @@ int sym_tile_r(const Tile& t, int n_pad)
 }
 
 size_t sym_ipart_floats(int n_pad, int K, int R) { (void)K; return (size_t)R * n_pad * 3; }
-size_t sym_jpart_floats(int n_pad, int K)
+size_t sym_jpart_floats(int n_pad, int K, int rs)
 {
     const int NW = n_pad / (64 * K);
-    return (size_t)NW * (NW / 2) * 64 * K * 3;
+    return (size_t)NW * (NW / 2) * rs * 64 * K * 3;
+}
+
+static int log2_exact(int v)
+{
+    int l = 0;
+    while ((1 << l) < v)
+        ++l;
+    return (1 << l) == v ? l : -1;
 }
 
 template <int K, bool COMP>
-static void launch_sym_pairs(bool newton, dim3 grid, const float4* src, int NW, int R, float eps2, float* ipart,
-                             float* jpart, hipStream_t st)
+static void launch_sym_pairs(bool newton, int rs, dim3 grid, const float4* src, int NW, int R, float eps2,
+                             float* ipart, float* jpart, hipStream_t st)
 {
+    if constexpr (K <= 4) {
+        if (rs > 1) {
+            if (newton)
+                hipLaunchKernelGGL((sym_force_kernel<K, true, COMP, true>), grid, dim3(256), 0, st, src, NW, R, eps2,
+                                   ipart, jpart, rs);
+            else
+                hipLaunchKernelGGL((sym_force_kernel<K, false, COMP, true>), grid, dim3(256), 0, st, src, NW, R, eps2,
+                                   ipart, jpart, rs);
+            return;
+        }
+    }
     if (newton)
-        hipLaunchKernelGGL((sym_force_kernel<K, true, COMP>), grid, dim3(256), 0, st, src, NW, R, eps2, ipart, jpart);
+        hipLaunchKernelGGL((sym_force_kernel<K, true, COMP, false>), grid, dim3(256), 0, st, src, NW, R, eps2,
+                           ipart, jpart, 1);
     else
-        hipLaunchKernelGGL((sym_force_kernel<K, false, COMP>), grid, dim3(256), 0, st, src, NW, R, eps2, ipart, jpart);
+        hipLaunchKernelGGL((sym_force_kernel<K, false, COMP, false>), grid, dim3(256), 0, st, src, NW, R, eps2,
+                           ipart, jpart, 1);
 }
 
 template <typename Acc, class Epi>
-static void launch_sym_reduce(int T, const float* ipart, const float* jpart, int n, int NW, int R, int K,
+static void launch_sym_reduce(int T, const float* ipart, const float* jpart, int n, int NW, int R, int K, int rsl,
                               const Epi& epi, hipStream_t st)
 {
     const dim3 rgrid((unsigned)(((long long)n * T + 255) / 256));
+#define SYM_REDUCE(TT) \
+    hipLaunchKernelGGL((sym_reduce_kernel<TT, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, rsl, epi)
     switch (T) {
-    case 1: hipLaunchKernelGGL((sym_reduce_kernel<1, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
-    case 2: hipLaunchKernelGGL((sym_reduce_kernel<2, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
-    case 4: hipLaunchKernelGGL((sym_reduce_kernel<4, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
-    case 8: hipLaunchKernelGGL((sym_reduce_kernel<8, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
-    default: hipLaunchKernelGGL((sym_reduce_kernel<16, Acc, Epi>), rgrid, dim3(256), 0, st, ipart, jpart, n, NW, R, K, epi); break;
+    case 1: SYM_REDUCE(1); break;
+    case 2: SYM_REDUCE(2); break;
+    case 4: SYM_REDUCE(4); break;
+    case 8: SYM_REDUCE(8); break;
+    default: SYM_REDUCE(16); break;
     }
+#undef SYM_REDUCE
 }
 
 // comp: Kahan-added block partials and an FP64 reduction (the MIXED step and
@@ -432,21 +528,21 @@ template <class Epi>
 hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, int n, int n_pad, float eps2,
                       float* ipart, float* jpart, const Epi& epi, hipStream_t st)
 {
-    const int K = t.k, R = sym_tile_r(t, n_pad);
-    if (n_pad % (128 * K) != 0 || !ipart || !jpart)
+    const int K = t.k, R = sym_tile_r(t, n_pad), rs = t.rs, rsl = log2_exact(rs);
+    if (n_pad % (128 * K) != 0 || !ipart || !jpart || rsl < 0 || rs > 16 || (rs > 1 && K > 4))
         return hipErrorInvalidValue;
     const int NW = n_pad / (64 * K);
     const int waves = NW * R;
     const dim3 grid((waves + 3) / 4);
     if (K == 2)
-        comp ? launch_sym_pairs<2, true>(newton, grid, src, NW, R, eps2, ipart, jpart, st)
-             : launch_sym_pairs<2, false>(newton, grid, src, NW, R, eps2, ipart, jpart, st);
+        comp ? launch_sym_pairs<2, true>(newton, rs, grid, src, NW, R, eps2, ipart, jpart, st)
+             : launch_sym_pairs<2, false>(newton, rs, grid, src, NW, R, eps2, ipart, jpart, st);
     else if (K == 4)
-        comp ? launch_sym_pairs<4, true>(newton, grid, src, NW, R, eps2, ipart, jpart, st)
-             : launch_sym_pairs<4, false>(newton, grid, src, NW, R, eps2, ipart, jpart, st);
+        comp ? launch_sym_pairs<4, true>(newton, rs, grid, src, NW, R, eps2, ipart, jpart, st)
+             : launch_sym_pairs<4, false>(newton, rs, grid, src, NW, R, eps2, ipart, jpart, st);
     else if (K == 8)
-        comp ? launch_sym_pairs<8, true>(newton, grid, src, NW, R, eps2, ipart, jpart, st)
-             : launch_sym_pairs<8, false>(newton, grid, src, NW, R, eps2, ipart, jpart, st);
+        comp ? launch_sym_pairs<8, true>(newton, rs, grid, src, NW, R, eps2, ipart, jpart, st)
+             : launch_sym_pairs<8, false>(newton, rs, grid, src, NW, R, eps2, ipart, jpart, st);
     else
         return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
@@ -454,9 +550,9 @@ hipError_t launch_sym(const Tile& t, bool newton, bool comp, const float4* src, 
         return e;
     const int T = sym_reduce_lanes(n);
     if (comp)
-        launch_sym_reduce<double>(T, ipart, jpart, n, NW, R, K, epi, st);
+        launch_sym_reduce<double>(T, ipart, jpart, n, NW, R, K, rsl, epi, st);
     else
-        launch_sym_reduce<float>(T, ipart, jpart, n, NW, R, K, epi, st);
+        launch_sym_reduce<float>(T, ipart, jpart, n, NW, R, K, rsl, epi, st);
     return hipGetLastError();
 }
 

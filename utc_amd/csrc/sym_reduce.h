@@ -1,11 +1,12 @@
 // sym_reduce.h -- per-body sums of the symmetric kernels' partials (device
 // code only), shared by the force (sym.hip) and VJP (sym_vjp.hip) reductions.
 //
-// Body b of a system with NW wave blocks of 64K bodies has Q = R + NW/2
+// Body b of a system with NW wave blocks of 64K bodies has Q = R + RS NW/2
 // partial records: q < R is wave r = q's target-side partial (ipart, R rows
-// of n_pad records), q = R + d the reaction from the wave block d + 1 places
-// below (jpart), absent for the block half way round when the upper half
-// skipped it.  T lanes share a body: lane t adds records t, t + T, t + 2T, ...
+// of n_pad records), q = R + RS d + c the reaction from rotation chunk c of
+// the wave block d + 1 places below (jpart, RS = 2^rsl chunks per sub-unit;
+// RS = 1 for the kernels without a rotation split), absent for the block half
+// way round when the upper half skipped it.  T lanes share a body: lane t adds records t, t + T, t + 2T, ...
 // in that order (in FP64 on the compensated paths, in FP32 on the plain one), then the T partial sums are combined by a butterfly
 // of __shfl_xor over fixed lane offsets.  The order depends only on (T, R,
 // NW), so results are bitwise repeatable; with T = 1 it is the plain
@@ -56,27 +57,27 @@ __device__ __forceinline__ void sym_reduce_lane(int& body, int& t)
 // body of the system for every lane.
 template <int T, typename P, typename Acc, int C>
 __device__ __forceinline__ void sym_split_sum(const P* __restrict__ ipart, const P* __restrict__ jpart, int b,
-                                              int NW, int R, int K, int t, Acc (&s)[C])
+                                              int NW, int R, int K, int t, Acc (&s)[C], int rsl = 0)
 {
     const int n_pad = NW * 64 * K;
     const int half = NW / 2, v = b / (64 * K), loc = b - v * (64 * K);
 #pragma unroll
     for (int c = 0; c < C; ++c)
         s[c] = Acc(0);
-    const int Q = R + half;
+    const int Q = R + (half << rsl);
 #pragma unroll 4
     for (int q = t; q < Q; q += T) {
         const P* p;
         if (q < R) {
             p = ipart + (size_t)q * n_pad + b;
         } else {
-            const int d = q - R;
+            const int dc = q - R, d = dc >> rsl;
             int w = v - 1 - d;
             if (w < 0)
                 w += NW;
             if (2 * (d + 1) == NW && w >= half)
                 continue;
-            p = jpart + ((size_t)w * half + d) * (64 * K) + loc;
+            p = jpart + ((((size_t)w * half + d) << rsl) + (dc - (d << rsl))) * (64 * K) + loc;
         }
         part_add(s, *p);
     }
