@@ -116,7 +116,10 @@ typedef struct {
     int32_t state_dtype; /* UTCNB_PREC_REFERENCE only: UTCNB_F32 or UTCNB_F64 */
     int32_t flags;       /* UTCNB_FLAG_* */
     int32_t tile;        /* force-kernel tile id 1..utcnb_tile_count() (e.g. from
-                            utcnb_autotune); 0 = built-in default */
+                            utcnb_autotune); 0 = built-in default.  A
+                            small-system tile (name smAtTLwWnCAPk) holds at
+                            most CAP bodies (N padded to a multiple of 1024):
+                            larger n gives UTCNB_EINVAL */
     double eps2;         /* softening squared, > 0 unless precision is REFERENCE */
     double damping;      /* velocity damping per step (reference default 1.0) */
     double dt;           /* time step */
