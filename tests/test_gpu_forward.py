@@ -5,6 +5,7 @@ reference's CPU path; the fast FP32 / mixed / FP64 paths must meet the stated
 tolerances against the FP64 oracle.
 """
 import hashlib
+import re
 import sys
 from pathlib import Path
 
@@ -271,13 +272,18 @@ def test_fp64_field_matches_cpu_fp64():
     assert np.abs(a - a64).max() <= 1e-12 * np.abs(a64).max()
 
 
+def small_cap(name):
+    """Bodies a small-system tile holds: its name ends in nCAPk (CAP/1024)."""
+    return int(re.search(r"n(\d+)k", name).group(1)) * 1024
+
+
 @pytest.mark.parametrize("prec", [U.PREC_FP32, U.PREC_MIXED])
 @pytest.mark.parametrize("tile", list(range(1, U.lib().utcnb_tile_count() + 1)))
 def test_every_tile_field_accuracy(prec, tile):
     n = 3000  # not a multiple of any tile: exercises clamped targets and padding
     name = U.tiles()[tile - 1]
     if name.startswith("sm"):  # small-system tiles hold at most CAP bodies (name ends nCAPk)
-        n = min(n, int(name.split("n")[-1][:-1]) * 1024 - 47)
+        n = min(n, small_cap(name) - 47)
     c = plummer(n, eps=0.005)
     a64 = O.accel_f64(c["pos"], c["eps"] ** 2)
     say(f"case: field prec={prec} tile={tile} n={n}")
@@ -290,7 +296,7 @@ def test_every_tile_field_accuracy(prec, tile):
 
 @pytest.mark.parametrize("name", [t for t in U.tiles() if t.startswith("sm")])
 def test_small_tiles_reject_systems_above_their_capacity(name):
-    cap = int(name.split("n")[-1][:-1]) * 1024
+    cap = small_cap(name)
     with U.System(cap, U.PREC_MIXED, 1e-4, 1e-3, tile=U.tiles().index(name) + 1):
         pass
     with pytest.raises(U.UtcnbError, match="holds at most"):
