@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_tiles():
     L = U.lib()
     assert L.utcnb_version().startswith(b"utcnb")
-    assert L.utcnb_tile_count() == 34
+    assert L.utcnb_tile_count() == 37
     assert U.tiles()[8] == "k2s8" and U.tiles()[12] == "sym8r4"
     assert L.utcnb_tile_name(0) == b"" and L.utcnb_tile_name(99) == b""
 

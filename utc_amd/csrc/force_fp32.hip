@@ -27,6 +27,8 @@ const Tile kTiles[] = {
     {1, 8, "sm1t4w8n2k", false, 1, 4, 2048}, {1, 8, "sm1t8w8n2k", false, 1, 8, 2048},
     {1, 16, "sm1t8w16n8k", false, 1, 8, 8192}, {1, 16, "sm1t16w16n4k", false, 1, 16, 4096},
     {1, 16, "sm1t16w16n8k", false, 1, 16, 8192}, {2, 16, "sm2t16w16n8k", false, 1, 16, 8192},
+    // more one-pass tiles (appended: ids are stable)
+    {2, 16, "k2s16", false}, {4, 8, "k4s8", false}, {4, 16, "k4s16", false},
 };
 }  // namespace
 
@@ -128,14 +130,25 @@ hipError_t launch_mixed_step(const StepArgs& a, const float4* snap_old, float4* 
     return launch_force<true, false>(a.tile, snap_old, a.n, a.n_pad, a.eps2f, epi, st);
 }
 
-// One-pass tile for a slice of nt targets against all sources: the symmetric
-// tiles evaluate unordered pairs of the whole system and do not split by
-// target, so a slice takes k2s8 when its tiles give every CU 64 waves and
-// k1s16 (twice the waves per target) otherwise.
+// One-pass tile for a slice of nt targets against all sources (the
+// symmetric tiles evaluate unordered pairs of the whole system and do not
+// split by target).  From a sweep on MI355X, slices of N = 262144, ms per
+// step (best three):
+//   nt =   8192: k1s16 1.81, k1s8 1.98, k2s16 2.86
+//   nt =  16384: k1s16 1.81, k1s8 2.12, k2s16 2.86
+//   nt =  32768: k2s16 2.89, k2s8 3.22, k1s16 3.59
+//   nt =  65536: k2s8 5.73, k2s16 5.84, k4s16 5.96   (k1s16 7.19)
+//   nt = 131072: k2s8 10.76, k2s16 11.74, k4s16 11.89 (k1s16 14.59)
+//   nt = 262144: k2s8 20.72, k2s16 23.65, k4s16 23.76
+// The fewer the targets per CU, the more source splits keep the waves busy.
 int slice_tile_default(int64_t nt, int cu_count)
 {
     const int64_t cus = cu_count > 0 ? cu_count : 256;
-    return (nt + 127) / 128 * 8 >= cus * 64 ? 8 : 9;
+    if (nt <= 64 * cus)
+        return 9;   // k1s16
+    if (nt <= 128 * cus)
+        return 34;  // k2s16
+    return 8;       // k2s8
 }
 
 hipError_t launch_fp32_step_slice(const StepArgs& a, int tile, int i0, int nt, const float4* pos_old,

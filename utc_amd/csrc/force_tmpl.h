@@ -314,6 +314,9 @@ hipError_t launch_force_range(int tile, const float4* src, int i0, int nt, int n
     case 31: return launch_small_tile<1, 16, 16, 4096, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
     case 32: return launch_small_tile<1, 16, 16, 8192, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
     case 33: return launch_small_tile<2, 16, 16, 8192, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 34: return launch_force_tile<2, 16, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 35: return launch_force_tile<4, 8, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
+    case 36: return launch_force_tile<4, 16, COMP, NEWTON>(src, i0, nt, n_pad, eps2, epi, st);
     default: return hipErrorInvalidValue;
     }
 }
