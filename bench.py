@@ -495,7 +495,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=262144,
                     help="rows of the CPU baseline's force evaluation (default: all, ~10 s on 16 threads)")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp32", "adjoint", "batched", "reference", "slice", "large", "small"],
+    ap.add_argument("--only", choices=["mixed", "mixed_newton", "fp32", "fp32_newton", "adjoint", "batched", "reference",
+                                       "slice", "large", "small"],
                     default=None,
                     help="time one path only and print its line (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",

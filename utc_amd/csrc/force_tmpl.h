@@ -15,10 +15,12 @@
 // Measured on MI355X this runs at ~99% of the non-packed FP32 VALU issue
 // rate; replacing y^3 by rsq(d2)*rcp(d2) (11 slots, two transcendentals) is
 // slower because the second transcendental does not co-issue.  The NEWTON
-// variant refines y with one Newton-Raphson step, written as
-// e = 1 - d2*y*y, y <- y + (y/2)*e (16 slots): the same step as
-// y*(1.5 - 0.5*d2*y*y), but it ran 26% faster in the symmetric kernel on
-// MI355X than that form (same-device A/B at N = 262144, 16.7 vs 21.0 ms).
+// variant applies one Newton-Raphson step to y, carried into the cube: with
+// e = 1 - d2*y^2 the refined y1 = y (1 + e/2) has y1^3 = y^3 (1 + 3e/2) +
+// O(e^2) y^3, and |e| ~ 2^-22 leaves the e^2 terms far below FP32 rounding,
+// so y1^3 = fma(y^3 * e, 1.5, y^3): three slots more than y^3 alone (15 per
+// pair).  Refining y first and cubing it (e, y/2, fma, then the cube) costs
+// four, and y*(1.5 - 0.5*d2*y*y) five.
 //
 // Compensation.  Each target keeps (hi, c) per component: 64 sources are
 // summed plainly into a partial, which is added to hi with Kahan's
@@ -35,10 +37,11 @@ template <bool NEWTON>
 __device__ __forceinline__ float pair_scale(float d2, float m)
 {
     if constexpr (NEWTON) {
-        float y = rsq_f32(d2);
-        const float e = __builtin_fmaf(-d2, y * y, 1.0f);
-        y = __builtin_fmaf(0.5f * y, e, y);
-        return m * y * y * y;
+        const float y = rsq_f32(d2);
+        const float y2 = y * y;
+        const float e = __builtin_fmaf(-d2, y2, 1.0f);
+        const float y3 = y2 * y;
+        return m * __builtin_fmaf(y3 * e, 1.5f, y3);
     } else {
         float y = rsq_f32(d2);
         return m * y * y * y;
