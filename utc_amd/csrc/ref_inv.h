@@ -11,13 +11,22 @@
 // neighbour.  (s0 - u and s0 + u are the neighbouring floats: the bit pattern
 // minus and plus one.)
 //
-// 1.0 / (double)s, rounded to float.  Double rounding is innocuous here, and
-// more: 1/s stays at least 2^-48 (relative) away from every float midpoint
-// (|1 - s m| >= 2^-48 for a float s and a midpoint m: s m is a 24-by-25-bit
-// product that cannot equal 1), so ANY FP64 approximation of 1/s within
-// 2^-49 rounds to the same float.  v_rcp_f64 and one Newton-Raphson step
-// y1 = y0 + y0 (1 - s y0) give that.  The clamp of s in the residual keeps
-// 1/inf = 0 (an overflowed d2) instead of inf * 0; NaN stays NaN.
+// 1.0 / (double)s, rounded to float.  1/s stays at least 2^-48 (relative)
+// away from every float midpoint (|1 - s m| >= 2^-48 for a float s and a
+// midpoint m: s m is a 24-by-25-bit product that cannot equal 1), so the
+// FP64 quotient (within 2^-53) rounds to the correctly rounded FP32
+// reciprocal of s.  That is computed in FP32: y0 = v_rcp_f32(s) (within 1
+// ulp), then one correction y1 = fma(y0, e, y0), e = fma(-s, y0, 1).  From a
+// correctly rounded y0 the step returns it for every s; from a y0 one ulp
+// off it returns the correctly rounded value except for 33 significands
+// (tests/test_ref_inv_cpu.py, in exact arithmetic), so exactness rests on
+// v_rcp_f32 not landing there: tests/device/probe_invdist.hip checks every
+// float d2 in [2^-96, inf] (every s this path meets) on the device, with no
+// mismatch on gfx950.  Against v_rcp_f64, a Newton step in FP64 and two
+// conversions, this saves a VALU slot and the FP64 transcendental per pair
+// (9% faster reference steps on MI355X).  The clamp of s in the
+// residual keeps 1/inf = 0 (an overflowed d2) instead of inf * 0; NaN stays
+// NaN.
 //
 // The compiler's generic sequences rescale tiny inputs; from 2^-96 up they
 // are not needed (s and 1/s are normal).  tests/device/probe_invdist.hip
@@ -47,10 +56,9 @@ __device__ __forceinline__ float ref_sqrt_rn(float x)
 
 __device__ __forceinline__ float ref_recip_rn(float s)
 {
-    const double d = (double)s;
-    const double y0 = __builtin_amdgcn_rcp(d);
-    const double e = __builtin_fma(-__builtin_fmin(d, 0x1p1000), y0, 1.0);
-    return (float)__builtin_fma(y0, e, y0);
+    const float y0 = __builtin_amdgcn_rcpf(s);
+    const float e = __builtin_fmaf(-__builtin_fminf(s, 0x1p127f), y0, 1.0f);
+    return __builtin_fmaf(y0, e, y0);
 }
 
 // (float)(1.0 / (double)sqrtf(d2)) for d2 >= kRefInvMin (or +inf, NaN).
