@@ -83,40 +83,44 @@ __device__ __forceinline__ void ref_pair(const T4& pj, const T4& pi, T eps2, T& 
     ref_accum(pj.w, inv_dist<T, FAST>(d2), r0, r1, r2, a0, a1, a2);
 }
 
-// Four FP64 sources with one wave-uniform choice: ref_inv_dist64 when every
-// lane's four d2 lie within its range (always, short of coordinates 1e90
+// G FP64 sources with one wave-uniform choice: ref_inv_dist64 when every
+// lane's G d2 lie within its range (always, short of coordinates 1e90
 // apart), else the generic sqrt and division.  Sums in j order as ref_pair.
-__device__ __forceinline__ void ref_quad64(const double4 (&q)[4], const double4& pi, double eps2,
-                                           double& a0, double& a1, double& a2)
+template <int G>
+__device__ __forceinline__ void ref_group64(const double4 (&q)[G], const double4& pi, double eps2,
+                                            double& a0, double& a1, double& a2)
 {
-    double r[4][3], d2[4], inv[4];
+    double r[G][3], d2[G], inv[G];
     bool ok = true;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < G; ++k) {
         d2[k] = ref_geom(q[k], pi, eps2, r[k][0], r[k][1], r[k][2]);
         ok = ok && d2[k] <= kRefInv64Max;
     }
     if (__ballot(!ok) == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < G; ++k)
             inv[k] = ref_inv_dist64(d2[k]);
     } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < G; ++k)
             inv[k] = 1.0 / sqrt(d2[k]);
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < G; ++k)
         ref_accum(q[k].w, inv[k], r[k][0], r[k][1], r[k][2], a0, a1, a2);
 }
 
-// Sources in index order, sixteen per trip for FP32 (four for FP64, whose
-// inverse distance takes one uniform choice per four sources): their loads
-// (one s_load_dwordx16 per four) and inverse distances are independent, so
-// they overlap -- with one thread per body that independence is the only
-// latency hiding at small N -- while the sums still take j = 0, 1, 2, ... in
-// turn (no reassociation in this file).  Sixteen ran 11% faster than four at
-// N = 262144 and 28% at N = 16384..65536 on MI355X; thirty-two spill SGPRs.
+// Sources in index order, sixteen per trip for FP32 (eight for FP64, whose
+// inverse distance takes one uniform choice per group): their loads (one
+// s_load_dwordx16 per four FP32 or two FP64 bodies) and inverse distances
+// are independent, so they overlap -- with one thread per body that
+// independence is the only latency hiding at small N -- while the sums still
+// take j = 0, 1, 2, ... in turn (no reassociation in this file).  On MI355X
+// sixteen FP32 sources ran 11% faster than four at N = 262144 and 28% at
+// N = 16384..65536 (thirty-two spill SGPRs); eight FP64 sources 5% faster
+// than four at 262144 and 20% at 16384..65536 (two VGPRs spill to scratch;
+// sixteen spill SGPRs and ran slower).
 template <typename T, typename T4, bool FAST>
 __device__ __forceinline__ void ref_field(const T4* __restrict__ pos, int n, T eps2, const T4& pi,
                                           T& a0, T& a1, T& a2)
@@ -133,12 +137,20 @@ __device__ __forceinline__ void ref_field(const T4* __restrict__ pos, int n, T e
             for (int k = 0; k < 16; ++k)
                 ref_pair<T, T4, FAST>(q[k], pi, eps2, a0, a1, a2);
         }
+    } else if constexpr (FAST) {
+        for (; j + 8 <= n; j += 8) {
+            T4 q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                q[k] = pos[j + k];
+            ref_group64<8>(q, pi, eps2, a0, a1, a2);
+        }
     }
     for (; j + 4 <= n; j += 4) {
         const T4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
         if constexpr (FAST && sizeof(T) == 8) {
             const T4 q[4] = {q0, q1, q2, q3};
-            ref_quad64(q, pi, eps2, a0, a1, a2);
+            ref_group64<4>(q, pi, eps2, a0, a1, a2);
         } else {
             ref_pair<T, T4, FAST>(q0, pi, eps2, a0, a1, a2);
             ref_pair<T, T4, FAST>(q1, pi, eps2, a0, a1, a2);
