@@ -303,6 +303,29 @@ def test_small_tiles_reject_systems_above_their_capacity(name):
         U.System(cap + 1, U.PREC_MIXED, 1e-4, 1e-3, tile=U.tiles().index(name) + 1)
 
 
+@pytest.mark.parametrize("name", ["sm1t8w16n8k", "sm2t16w16n8k", "sym4r64x2", "sym2r128x4"])
+def test_small_and_chunked_rotation_tiles_are_repeatable(name):
+    """Fixed summation orders: two handles stepping the same state give
+    bitwise identical trajectories (mixed and plain FP32), and the small-system
+    tiles' field matches the CPU FP64 oracle at a ragged size."""
+    n = 5003
+    c = plummer(n, eps=0.005)
+    tile = U.tiles().index(name) + 1
+    say(f"case: repeatability tile={name} n={n}")
+    for prec in (U.PREC_MIXED, U.PREC_FP32):
+        out = []
+        for _ in range(2):
+            with U.System(n, prec, c["eps"] ** 2, c["dt"], tile=tile) as s:
+                s.set_state(c["pos"], c["vel"])
+                s.step(5)
+                out.append(s.get_state())
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), prec
+    a64 = O.accel_f64(c["pos"], c["eps"] ** 2)
+    with U.System(n, U.PREC_MIXED, c["eps"] ** 2, c["dt"], tile=tile) as s:
+        s.set_state(c["pos"], c["vel"])
+        assert relnorm(s.accel(), a64) <= 2e-6
+
+
 @pytest.mark.parametrize("pattern", ["equal", "one_heavy", "few_heavy", "loguniform"])
 @pytest.mark.parametrize("name", ["sym2r32", "sym4r32", "sym8r32", "sym8r64", "sym8r4", "sym2r128x4", "sym4r64x2",
                                   "sym4r128x4"])
