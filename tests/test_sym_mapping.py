@@ -101,3 +101,80 @@ def test_reduction_lanes_cover_every_body(T):
             off //= 2
         assert sum(offs) == 64 - 64 // T  # together they span the t bits
     assert reduce_lanes(262144) == 2 and reduce_lanes(16384) == 16 and reduce_lanes(1 << 19) == 1
+
+
+def chunk_cover(NW, R, K, rs):
+    """Rotation chunks (sym_wave with SPLIT): sub-unit q of wave block w is cut
+    into rs chunks c of 64/rs rotation steps; chunk c starts with lane l
+    holding source (l - t0) mod 64 (t0 = 64 c / rs) and, at its step t, lane l
+    meets source (l - t) mod 64; a two-sided chunk stores the reaction sums that
+    lane l then holds, those of source (l - t0 - steps) mod 64, to record
+    ((w NW/2 + d) rs + c) 64K + kb 64 + that source; an own-block chunk shuffles
+    them home from lane (l + rot) mod 64, rot = (t0 + steps) mod 64.  Counts,
+    per (target body, source body), the target-side and the reaction
+    evaluations, and the reaction records written and read (sym_split_sum with
+    rsl = log2 rs)."""
+    half = NW // 2
+    Q = (half + 1) * K
+    QS = Q * rs
+    steps = 64 // rs
+    B = 64 * K
+    side, react, writes = {}, {}, {}
+    for w in range(NW):
+        for r in range(R):
+            for qs in range(QS * r // R, QS * (r + 1) // R):
+                q, c = divmod(qs, rs)
+                u, kb = divmod(q, K)
+                t0 = c * steps
+                if u == 0:
+                    for lane in range(64):
+                        for t in range(t0, t0 + steps):
+                            s = w * B + kb * 64 + (lane - t) % 64
+                            for k in range(kb + 1):  # slots k < kb two-sided, slot kb target side
+                                i = w * B + k * 64 + lane
+                                side[(i, s)] = side.get((i, s), 0) + 1
+                                if k < kb:
+                                    react[(s, i)] = react.get((s, i), 0) + 1
+                    rot = (t0 + steps) % 64
+                    for h in range(64):  # lane h receives the sums of lane (h + rot): source h
+                        assert ((h + rot) % 64 - t0 - steps) % 64 == h
+                    continue
+                d = u - 1
+                if 2 * (d + 1) == NW and w >= half:
+                    continue
+                v = (w + 1 + d) % NW
+                for lane in range(64):
+                    for t in range(t0, t0 + steps):
+                        s = v * B + kb * 64 + (lane - t) % 64
+                        for k in range(K):
+                            i = w * B + k * 64 + lane
+                            side[(i, s)] = side.get((i, s), 0) + 1
+                            react[(s, i)] = react.get((s, i), 0) + 1
+                    el = (lane - t0 - steps) % 64
+                    rec = ((w * half + d) * rs + c) * B + kb * 64 + el
+                    writes[rec] = writes.get(rec, 0) + 1
+    reads = {}
+    for b in range(NW * B):
+        v, loc = divmod(b, B)
+        for dc in range(half * rs):
+            d, c = divmod(dc, rs)
+            w = (v - 1 - d) % NW
+            if 2 * (d + 1) == NW and w >= half:
+                continue
+            rec = ((w * half + d) * rs + c) * B + loc
+            reads[rec] = reads.get(rec, 0) + 1
+    return side, react, writes, reads
+
+
+@pytest.mark.parametrize("rs", [1, 2, 4, 8])
+@pytest.mark.parametrize("K,R", [(2, 1), (2, 5), (4, 3), (4, 16)])
+def test_rotation_chunks_evaluate_every_ordered_pair_once(K, R, rs):
+    for NW in (1, 2, 3, 4):
+        side, react, writes, reads = chunk_cover(NW, R, K, rs)
+        n = NW * 64 * K
+        both = {}
+        for key, cnt in list(side.items()) + list(react.items()):
+            both[key] = both.get(key, 0) + cnt
+        assert len(both) == n * n, (NW, R, K, rs)
+        assert all(cnt == 1 for cnt in both.values()), (NW, R, K, rs)
+        assert all(cnt == 1 for cnt in writes.values()) and reads == writes, (NW, R, K, rs)
