@@ -57,7 +57,7 @@ extern "C" {
 
 /* config flags */
 #define UTCNB_FLAG_NO_NEWTON 1 /* FP32/MIXED: use v_rsq_f32 without the Newton step
-                                  (same measured accuracy, ~20% faster) */
+                                  (same measured accuracy, ~15% faster) */
 #define UTCNB_FLAG_SYM_CHUNKS 2 /* MIXED with a symmetric tile: evaluate each step in
                                    chunks of wave blocks (partial-sum memory for one
                                    chunk's reactions, one FP64 pass more); taken
